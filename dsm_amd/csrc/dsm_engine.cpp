@@ -1,0 +1,976 @@
+// Host side of the MI355X continuous subgraph matching engine: the C ABI of
+// include/dsm.h over the kernels of dsm_kernels.hip.
+//
+// Per batch (dsm_apply_batch + dsm_match_batch):
+//   1. delete records seed DFS roots in the pre-batch graph (negatives)
+//   2. host: symmetrise, stable (src,dst) sort, final op per key, final label
+//      per vertex (the update rule of cpuGraph.cpp:521-607)
+//   3. device: merge/commit the per-vertex deltas in place, relabel, refresh
+//      the candidate bitmaps of the dirty vertices
+//   4. insert records seed DFS roots in the post-batch graph (positives)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/dsm.h"
+#include "dsm_internal.h"
+
+using namespace dsm;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct QueryHost {
+  uint32_t n = 0;
+  uint32_t lab[kMaxQ] = {};
+  uint32_t adj[kMaxQ] = {};  // bitmask
+};
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    reset();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  hipError_t ensure(size_t count) { return count <= n ? hipSuccess : alloc(count + count / 2); }
+};
+
+template <typename T>
+struct PinnedBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~PinnedBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t ensure(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    reset();
+    size_t c = count + count / 2 + 16;
+    hipError_t e = hipHostMalloc(&p, c * sizeof(T), hipHostMallocDefault);
+    if (e == hipSuccess) n = c;
+    return e;
+  }
+};
+
+uint8_t clamp_label(uint32_t l, const bool used[256]) { return (l < 255 && used[l]) ? (uint8_t)l : kNoLabel; }
+
+}  // namespace
+
+struct dsm_engine {
+  dsm_options opt{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  uint32_t n = 0;
+  std::vector<QueryHost> queries;
+  bool label_used[256] = {};
+
+  // candidate signatures and plans
+  std::vector<SigReq> sigs;
+  std::vector<uint32_t> sig_lo, sig_hi;  // 256
+  std::vector<uint8_t> lmap;             // 256
+  std::vector<std::vector<uint16_t>> qsig;  // per query, per query vertex
+  std::vector<Plan> plans;
+  std::vector<uint32_t> plan_key;
+  uint32_t max_plans_per_key = 0;
+
+  // device graph
+  DevBuf<uint32_t> pool, start, deg, cap;
+  DevBuf<uint8_t> lab;
+  unsigned long long pool_cap = 0;
+  // filter
+  DevBuf<uint64_t> bm;
+  uint32_t words = 0;
+  DevBuf<SigReq> d_sigs;
+  DevBuf<uint32_t> d_sig_lo, d_sig_hi;
+  DevBuf<uint8_t> d_lmap;
+  DevBuf<Plan> d_plans;
+  DevBuf<uint32_t> d_plan_key;
+  // batch
+  DevBuf<uint32_t> scratch;
+  unsigned long long scratch_cap = 0;
+  DevBuf<uint32_t> dirty_bits, dirty_list, heavy_list;
+  DevBuf<uint32_t> bbuf;              // batch arrays (device)
+  PinnedBuf<uint32_t> hbuf;           // batch arrays (pinned staging)
+  DevBuf<uint32_t> ndeg, fail;
+  DevBuf<unsigned long long> soff;
+  DevBuf<ApplyCtl> actl;
+  PinnedBuf<ApplyCtl> h_actl;
+  // DFS
+  DevBuf<DfsCtl> ctl;
+  PinnedBuf<DfsCtl> h_ctl;
+  DevBuf<Task> queue;
+  DevBuf<uint32_t> ready;
+  uint32_t qcap = 0, epoch = 0;
+  DevBuf<uint32_t> seed_plan, seed_s, seed_d;
+  uint32_t seed_cap = 0;
+  DevBuf<unsigned long long> counts;
+  PinnedBuf<unsigned long long> h_counts;
+  DevBuf<unsigned long long> tuple_count;
+  uint32_t *tuples = nullptr;
+  bool tuples_host = false;
+  unsigned long long tuple_cap = 0;
+  uint32_t dfs_blocks = 0, dfs_wpb = 4;
+  // records of the last batch (file order)
+  std::vector<uint32_t> pos_s, pos_d, neg_s, neg_d;
+  DevBuf<uint32_t> rec_neg, rec_pos;  // records on the device: src[k] | dst[k]
+  bool batch_pending = false;
+  hipEvent_t ev[12] = {};
+  dsm_stats stats{};
+
+  ~dsm_engine() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (tuples) {
+      if (tuples_host) (void)hipHostFree(tuples);
+      else (void)hipFree(tuples);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int fail_msg(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+
+  DevGraph dev_graph() {
+    DevGraph g;
+    g.n = n;
+    g.pool = pool.p;
+    g.start = start.p;
+    g.deg = deg.p;
+    g.cap = cap.p;
+    g.lab = lab.p;
+    return g;
+  }
+  FilterTables tables() {
+    FilterTables t;
+    t.sigs = d_sigs.p;
+    t.sig_lo = d_sig_lo.p;
+    t.sig_hi = d_sig_hi.p;
+    t.lmap = d_lmap.p;
+    t.nsig = (uint32_t)sigs.size();
+    t.bm = bm.p;
+    t.words = words;
+    return t;
+  }
+  DirtyArgs dirty() {
+    DirtyArgs d;
+    d.bits = dirty_bits.p;
+    d.list = dirty_list.p;
+    d.heavy = heavy_list.p;
+    d.list_cap = n;
+    d.actl = actl.p;
+    return d;
+  }
+};
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess)                                                                     \
+      return e->fail_msg(DSM_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                         __FILE__, __LINE__);                                                 \
+  } while (0)
+
+namespace {
+
+// ------------------------------------------------------------ query set-up
+
+int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq) {
+  if (nq == 0 || nq > DSM_MAX_QUERIES) return e->fail_msg(DSM_EINVAL, "need 1..%d queries", DSM_MAX_QUERIES);
+  e->queries.resize(nq);
+  for (uint32_t q = 0; q < nq; q++) {
+    const dsm_query_desc &d = qs[q];
+    QueryHost &Q = e->queries[q];
+    if (d.num_vertices < 2 || d.num_vertices > (uint32_t)kMaxQ)
+      return e->fail_msg(DSM_ELIMIT, "query %u: %u vertices (2..%d supported)", q, d.num_vertices, kMaxQ);
+    Q.n = d.num_vertices;
+    for (uint32_t u = 0; u < Q.n; u++) {
+      if (d.labels[u] > DSM_MAX_LABEL)
+        return e->fail_msg(DSM_ELIMIT, "query %u: label %u > %d", q, d.labels[u], DSM_MAX_LABEL);
+      Q.lab[u] = d.labels[u];
+      e->label_used[d.labels[u]] = true;
+    }
+    for (uint32_t i = 0; i < d.num_edges; i++) {
+      uint32_t a = d.edge_src[i], b = d.edge_dst[i];
+      if (a >= Q.n || b >= Q.n || a == b) return e->fail_msg(DSM_EINVAL, "query %u: bad edge %u-%u", q, a, b);
+      Q.adj[a] |= 1u << b;
+      Q.adj[b] |= 1u << a;
+    }
+    // connected (every later DFS position needs an earlier neighbour)
+    uint32_t seen = 1, frontier = 1;
+    while (frontier) {
+      uint32_t nf = 0;
+      for (uint32_t u = 0; u < Q.n; u++)
+        if (frontier >> u & 1) nf |= Q.adj[u];
+      frontier = nf & ~seen;
+      seen |= nf;
+    }
+    if (seen != ((1u << Q.n) - 1)) return e->fail_msg(DSM_EINVAL, "query %u is not connected", q);
+  }
+  // relevant labels: those of query vertices with a neighbour (all of them)
+  e->lmap.assign(256, 0xFF);
+  uint32_t nrel = 0;
+  for (uint32_t l = 0; l < 255; l++)
+    if (e->label_used[l]) {
+      if (nrel == (uint32_t)kMaxRel)
+        return e->fail_msg(DSM_ELIMIT, "queries use more than %d distinct labels", kMaxRel);
+      e->lmap[l] = (uint8_t)nrel++;
+    }
+  // signatures: (label, degree, neighbour-label counts) of every query vertex
+  std::map<std::vector<uint64_t>, uint32_t> sigidx;
+  std::vector<SigReq> tmp;
+  e->qsig.assign(nq, std::vector<uint16_t>());
+  for (uint32_t q = 0; q < nq; q++) {
+    const QueryHost &Q = e->queries[q];
+    e->qsig[q].resize(Q.n);
+    for (uint32_t u = 0; u < Q.n; u++) {
+      SigReq r{};
+      r.label = Q.lab[u];
+      r.deg = (uint32_t)__builtin_popcount(Q.adj[u]);
+      uint8_t cnt[kMaxRel] = {};
+      for (uint32_t w = 0; w < Q.n; w++)
+        if (Q.adj[u] >> w & 1) cnt[e->lmap[Q.lab[w]]]++;
+      for (int l = 0; l < kMaxRel; l++) r.nlc[l >> 3] |= (uint64_t)cnt[l] << ((l & 7) * 8);
+      std::vector<uint64_t> key = {r.label, r.deg, r.nlc[0], r.nlc[1], r.nlc[2], r.nlc[3]};
+      auto it = sigidx.find(key);
+      if (it == sigidx.end()) {
+        it = sigidx.emplace(key, (uint32_t)tmp.size()).first;
+        tmp.push_back(r);
+      }
+      e->qsig[q][u] = (uint16_t)it->second;
+    }
+  }
+  if (tmp.size() > 65535) return e->fail_msg(DSM_ELIMIT, "too many distinct query vertex signatures");
+  // sort signatures by label; remap
+  std::vector<uint32_t> ord(tmp.size());
+  for (uint32_t i = 0; i < ord.size(); i++) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return tmp[a].label < tmp[b].label; });
+  std::vector<uint32_t> newidx(tmp.size());
+  e->sigs.resize(tmp.size());
+  for (uint32_t i = 0; i < ord.size(); i++) {
+    e->sigs[i] = tmp[ord[i]];
+    newidx[ord[i]] = i;
+  }
+  for (auto &v : e->qsig)
+    for (auto &s : v) s = (uint16_t)newidx[s];
+  e->sig_lo.assign(256, 0);
+  e->sig_hi.assign(256, 0);
+  for (uint32_t i = 0; i < e->sigs.size(); i++) {
+    uint32_t l = e->sigs[i].label;
+    if (e->sig_hi[l] == 0) e->sig_lo[l] = i;
+    e->sig_hi[l] = i + 1;
+  }
+  // plans: every query edge in both orientations
+  struct PK {
+    uint32_t key;
+    Plan p;
+  };
+  std::vector<PK> pk;
+  for (uint32_t q = 0; q < nq; q++) {
+    const QueryHost &Q = e->queries[q];
+    for (uint32_t u = 0; u < Q.n; u++)
+      for (uint32_t v = 0; v < Q.n; v++) {
+        if (!(Q.adj[u] >> v & 1)) continue;
+        Plan P{};
+        P.n = (uint8_t)Q.n;
+        P.query = (uint16_t)q;
+        uint32_t order[kMaxQ];
+        order[0] = u;
+        order[1] = v;
+        uint32_t placed = (1u << u) | (1u << v);
+        for (uint32_t k = 2; k < Q.n; k++) {
+          // most earlier neighbours, then highest query degree, then lowest id
+          int best = -1, bb = -1, bd = -1;
+          for (uint32_t w = 0; w < Q.n; w++) {
+            if (placed >> w & 1) continue;
+            int b = __builtin_popcount(Q.adj[w] & placed), dg = __builtin_popcount(Q.adj[w]);
+            if (b > bb || (b == bb && dg > bd)) {
+              best = (int)w;
+              bb = b;
+              bd = dg;
+            }
+          }
+          order[k] = (uint32_t)best;
+          placed |= 1u << best;
+        }
+        for (uint32_t k = 0; k < Q.n; k++) {
+          P.qv[k] = (uint8_t)order[k];
+          P.sig[k] = e->qsig[q][order[k]];
+          uint16_t back = 0, inj = 0;
+          for (uint32_t j = 0; j < k; j++) {
+            if (Q.adj[order[k]] >> order[j] & 1) back |= (uint16_t)(1u << j);
+            if (Q.lab[order[j]] == Q.lab[order[k]]) inj |= (uint16_t)(1u << j);
+          }
+          P.back[k] = back;
+          P.inj[k] = inj;
+        }
+        pk.push_back({(Q.lab[u] << 8) | Q.lab[v], P});
+      }
+  }
+  std::stable_sort(pk.begin(), pk.end(), [](const PK &a, const PK &b) { return a.key < b.key; });
+  e->plans.clear();
+  e->plan_key.clear();
+  e->max_plans_per_key = 0;
+  for (size_t i = 0; i < pk.size();) {
+    size_t j = i;
+    while (j < pk.size() && pk[j].key == pk[i].key) j++;
+    e->max_plans_per_key = std::max<uint32_t>(e->max_plans_per_key, (uint32_t)(j - i));
+    i = j;
+  }
+  for (auto &x : pk) {
+    e->plans.push_back(x.p);
+    e->plan_key.push_back(x.key);
+  }
+  return DSM_OK;
+}
+
+// ------------------------------------------------------------- graph set-up
+
+int upload_graph(dsm_engine *e, const dsm_graph_desc *g) {
+  const uint32_t n = g->num_vertices;
+  // sorted, deduplicated CSR
+  std::vector<uint64_t> off(n + 1, 0);
+  for (uint64_t i = 0; i < g->num_edges; i++) {
+    uint32_t a = g->edge_src[i], b = g->edge_dst[i];
+    if (a >= n || b >= n) return e->fail_msg(DSM_EINVAL, "edge %llu: endpoint out of range", (unsigned long long)i);
+    off[a + 1]++;
+    off[b + 1]++;
+  }
+  for (uint32_t v = 0; v < n; v++) off[v + 1] += off[v];
+  std::vector<uint32_t> nb(off[n]);
+  {
+    std::vector<uint64_t> pos(off.begin(), off.end() - 1);
+    for (uint64_t i = 0; i < g->num_edges; i++) {
+      uint32_t a = g->edge_src[i], b = g->edge_dst[i];
+      nb[pos[a]++] = b;
+      nb[pos[b]++] = a;
+    }
+  }
+  std::vector<uint32_t> dg(n), cp(n), st(n);
+  unsigned long long total = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    auto b = nb.begin() + off[v], en = nb.begin() + off[v + 1];
+    std::sort(b, en);
+    dg[v] = (uint32_t)(std::unique(b, en) - b);
+    cp[v] = dg[v] + dg[v] / 4 + 4;
+    st[v] = (uint32_t)total;
+    total += cp[v];
+  }
+  const unsigned long long pcap = total + total / 2 + (1ull << 20);
+  if (pcap >= (1ull << 32)) return e->fail_msg(DSM_ELIMIT, "graph too large for 32-bit pool offsets");
+  std::vector<uint32_t> hp(total, 0xFFFFFFFFu);
+  for (uint32_t v = 0; v < n; v++) std::copy(nb.begin() + off[v], nb.begin() + off[v] + dg[v], hp.begin() + st[v]);
+  std::vector<uint8_t> hl(n);
+  for (uint32_t v = 0; v < n; v++) hl[v] = clamp_label(g->labels[v], e->label_used);
+
+  HIPCHK(e->pool.alloc(pcap));
+  HIPCHK(e->start.alloc(n));
+  HIPCHK(e->deg.alloc(n));
+  HIPCHK(e->cap.alloc(n));
+  HIPCHK(e->lab.alloc(n));
+  e->pool_cap = pcap;
+  HIPCHK(hipMemcpy(e->pool.p, hp.data(), total * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->start.p, st.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->deg.p, dg.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->cap.p, cp.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->lab.p, hl.data(), n, hipMemcpyHostToDevice));
+  // scratch holds one merged copy of every touched list
+  e->scratch_cap = pcap + (1ull << 20);
+  HIPCHK(e->scratch.alloc(e->scratch_cap));
+  HIPCHK(e->actl.alloc(1));
+  HIPCHK(e->h_actl.ensure(1));
+  ApplyCtl a0{};
+  a0.pool_used = total;
+  HIPCHK(hipMemcpy(e->actl.p, &a0, sizeof a0, hipMemcpyHostToDevice));
+  HIPCHK(e->dirty_bits.alloc((n + 31) / 32));
+  HIPCHK(hipMemset(e->dirty_bits.p, 0, ((n + 31) / 32) * 4));
+  HIPCHK(e->dirty_list.alloc(n));
+  HIPCHK(e->heavy_list.alloc(n));
+  return DSM_OK;
+}
+
+int upload_tables(dsm_engine *e) {
+  e->words = (e->n + 63) / 64;
+  HIPCHK(e->bm.alloc((size_t)e->sigs.size() * e->words));
+  HIPCHK(hipMemset(e->bm.p, 0, (size_t)e->sigs.size() * e->words * 8));
+  HIPCHK(e->d_sigs.alloc(e->sigs.size()));
+  HIPCHK(hipMemcpy(e->d_sigs.p, e->sigs.data(), e->sigs.size() * sizeof(SigReq), hipMemcpyHostToDevice));
+  HIPCHK(e->d_sig_lo.alloc(256));
+  HIPCHK(e->d_sig_hi.alloc(256));
+  HIPCHK(e->d_lmap.alloc(256));
+  HIPCHK(hipMemcpy(e->d_sig_lo.p, e->sig_lo.data(), 256 * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->d_sig_hi.p, e->sig_hi.data(), 256 * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->d_lmap.p, e->lmap.data(), 256, hipMemcpyHostToDevice));
+  HIPCHK(e->d_plans.alloc(e->plans.size()));
+  HIPCHK(hipMemcpy(e->d_plans.p, e->plans.data(), e->plans.size() * sizeof(Plan), hipMemcpyHostToDevice));
+  HIPCHK(e->d_plan_key.alloc(e->plan_key.size()));
+  HIPCHK(hipMemcpy(e->d_plan_key.p, e->plan_key.data(), e->plan_key.size() * 4, hipMemcpyHostToDevice));
+  const uint32_t nq = (uint32_t)e->queries.size();
+  HIPCHK(e->counts.alloc((size_t)nq * 2));
+  HIPCHK(e->h_counts.ensure((size_t)nq * 2));
+  HIPCHK(e->ctl.alloc(1));
+  HIPCHK(e->h_ctl.ensure(1));
+  e->qcap = e->opt.queue_capacity ? e->opt.queue_capacity : (1u << 20);
+  HIPCHK(e->queue.alloc(e->qcap));
+  HIPCHK(e->ready.alloc(e->qcap));
+  HIPCHK(hipMemset(e->ready.p, 0, (size_t)e->qcap * 4));
+  HIPCHK(e->tuple_count.alloc(1));
+  if (e->opt.tuple_capacity) {
+    e->tuple_cap = e->opt.tuple_capacity;
+    size_t bytes = (size_t)e->tuple_cap * (kMaxQ + 2) * 4;
+    if (e->opt.tuples_in_host_memory) {
+      HIPCHK(hipHostMalloc((void **)&e->tuples, bytes, hipHostMallocMapped));
+      e->tuples_host = true;
+    } else {
+      HIPCHK(hipMalloc((void **)&e->tuples, bytes));
+    }
+  }
+  HIPCHK(e->ndeg.alloc(1024));
+  HIPCHK(e->fail.alloc(1024));
+  HIPCHK(e->soff.alloc(1024));
+  return DSM_OK;
+}
+
+// ------------------------------------------------------------- DFS rounds
+
+int ensure_seed_cap(dsm_engine *e, uint32_t nrec) {
+  const uint64_t need = (uint64_t)nrec * std::max<uint32_t>(e->max_plans_per_key, 1);
+  if (need > 0xFFFFFFF0ull) return e->fail_msg(DSM_ELIMIT, "seed table too large");
+  if (need > e->seed_cap) {
+    uint32_t c = (uint32_t)std::min<uint64_t>(need + need / 2 + 1024, 0xFFFFFFF0ull);
+    HIPCHK(e->seed_plan.alloc(c));
+    HIPCHK(e->seed_s.alloc(c));
+    HIPCHK(e->seed_d.alloc(c));
+    e->seed_cap = c;
+  }
+  return DSM_OK;
+}
+
+// Seeds the records (device arrays rs/rd) and runs the persistent DFS.
+int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nrec, uint32_t sign,
+              hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
+  HIPCHK(hipEventRecord(ev0, e->stream));
+  HIPCHK(launch_ctl_reset(e->ctl.p, nullptr, nullptr, nullptr, 0, e->stream));
+  SeedArgs sa;
+  sa.g = e->dev_graph();
+  sa.bm = e->bm.p;
+  sa.words = e->words;
+  sa.plans = e->d_plans.p;
+  sa.plan_key = e->d_plan_key.p;
+  sa.nplans = (uint32_t)e->plans.size();
+  sa.rs = rs;
+  sa.rd = rd;
+  sa.nrec = nrec;
+  sa.seed_plan = e->seed_plan.p;
+  sa.seed_s = e->seed_s.p;
+  sa.seed_d = e->seed_d.p;
+  sa.seed_cap = e->seed_cap;
+  sa.ctl = e->ctl.p;
+  HIPCHK(launch_seed(sa, e->stream));
+  HIPCHK(hipEventRecord(ev1, e->stream));
+  DfsArgs da;
+  da.g = e->dev_graph();
+  da.bm = e->bm.p;
+  da.words = e->words;
+  da.plans = e->d_plans.p;
+  da.seed_plan = e->seed_plan.p;
+  da.seed_s = e->seed_s.p;
+  da.seed_d = e->seed_d.p;
+  da.queue = e->queue.p;
+  da.ready = e->ready.p;
+  da.qcap = e->qcap;
+  da.epoch = ++e->epoch;
+  da.donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
+  da.sign = sign;
+  da.ctl = e->ctl.p;
+  da.counts = e->counts.p;
+  da.tuples = e->tuples;
+  da.tuple_cap = e->tuple_cap;
+  da.tuple_count = e->tuple_count.p;
+  HIPCHK(launch_dfs(da, e->dfs_blocks, e->dfs_wpb, e->stream));
+  HIPCHK(hipEventRecord(ev2, e->stream));
+  HIPCHK(hipMemcpyAsync(e->h_ctl.p, e->ctl.p, sizeof(DfsCtl), hipMemcpyDeviceToHost, e->stream));
+  return DSM_OK;
+}
+
+int check_round(dsm_engine *e, uint64_t *seeds) {
+  const DfsCtl &c = *e->h_ctl.p;
+  if (c.seed_overflow) return e->fail_msg(DSM_ELIMIT, "seed table overflow (%u seeds dropped)", c.seed_overflow);
+  *seeds = c.nseeds;
+  e->stats.tasks_donated += c.donated;
+  return DSM_OK;
+}
+
+// ------------------------------------------------------------ apply rounds
+
+struct HostDelta {
+  std::vector<uint32_t> tv, ioff, doff, ins, del, rl_v, rl_l;
+};
+
+// The update rule of cpuGraph.cpp:521-607 reduced to its net effect: records
+// symmetrised, stably sorted by (src,dst) (original records before reversed
+// copies on ties), each directed key ends in the state of its LAST record, and
+// each vertex's label becomes the label of the last insert record with that
+// vertex as src.
+void host_delta(const dsm_engine *e, const dsm_change *ch, uint32_t k, HostDelta &d) {
+  struct R {
+    uint32_t s, t, add, label, pos;
+  };
+  std::vector<R> r(2 * (size_t)k);
+  for (uint32_t i = 0; i < k; i++) {
+    r[i] = {ch[i].src, ch[i].dst, ch[i].add ? 1u : 0u, ch[i].label, i};
+    r[k + i] = {ch[i].dst, ch[i].src, ch[i].add ? 1u : 0u, ch[i].label, k + i};
+  }
+  std::sort(r.begin(), r.end(), [](const R &a, const R &b) {
+    if (a.s != b.s) return a.s < b.s;
+    if (a.t != b.t) return a.t < b.t;
+    return a.pos < b.pos;
+  });
+  d = HostDelta();
+  d.ioff.push_back(0);
+  d.doff.push_back(0);
+  for (size_t i = 0; i < r.size();) {
+    const uint32_t s = r[i].s;
+    size_t j = i;
+    int last_add = -1;
+    while (j < r.size() && r[j].s == s) {
+      size_t k2 = j;
+      while (k2 + 1 < r.size() && r[k2 + 1].s == s && r[k2 + 1].t == r[j].t) k2++;
+      if (r[k2].add) d.ins.push_back(r[j].t);
+      else d.del.push_back(r[j].t);
+      for (size_t x = j; x <= k2; x++)
+        if (r[x].add) last_add = (int)x;
+      j = k2 + 1;
+    }
+    d.tv.push_back(s);
+    d.ioff.push_back((uint32_t)d.ins.size());
+    d.doff.push_back((uint32_t)d.del.size());
+    if (e->opt.relabel && last_add >= 0) {
+      d.rl_v.push_back(s);
+      d.rl_l.push_back(clamp_label(r[last_add].label, e->label_used));
+    }
+    i = j;
+  }
+}
+
+// Uploads one apply round (a slice of every touched vertex's lists) and runs
+// merge + commit. Grows the pool when a commit found no room.
+int grow_pool(dsm_engine *e, unsigned long long extra);
+
+int apply_round(dsm_engine *e, const HostDelta &d, const std::vector<uint32_t> &tv,
+                const std::vector<uint32_t> &ioff, const std::vector<uint32_t> &doff,
+                const std::vector<uint32_t> &ins, const std::vector<uint32_t> &del) {
+  const uint32_t T = (uint32_t)tv.size();
+  if (T == 0) return DSM_OK;
+  (void)d;
+  // layout: tv | ioff | doff | ins | del
+  const size_t o_tv = 0, o_io = o_tv + T, o_do = o_io + T + 1, o_in = o_do + T + 1, o_de = o_in + ins.size();
+  const size_t total = o_de + del.size();
+  // queued work must be done with the staging and device buffers before they are reused
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(e->hbuf.ensure(total));
+  HIPCHK(e->bbuf.ensure(total));
+  uint32_t *h = e->hbuf.p;
+  std::copy(tv.begin(), tv.end(), h + o_tv);
+  std::copy(ioff.begin(), ioff.end(), h + o_io);
+  std::copy(doff.begin(), doff.end(), h + o_do);
+  std::copy(ins.begin(), ins.end(), h + o_in);
+  std::copy(del.begin(), del.end(), h + o_de);
+  HIPCHK(hipMemcpyAsync(e->bbuf.p, h, total * 4, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(e->ndeg.ensure(T));
+  HIPCHK(e->fail.ensure(T));
+  HIPCHK(e->soff.ensure(T));
+  BatchDev b{};
+  b.ntouched = T;
+  b.tv = e->bbuf.p + o_tv;
+  b.ioff = e->bbuf.p + o_io;
+  b.doff = e->bbuf.p + o_do;
+  b.ins = e->bbuf.p + o_in;
+  b.del = e->bbuf.p + o_de;
+  b.ndeg = e->ndeg.p;
+  b.soff = e->soff.p;
+  b.fail = e->fail.p;
+  // scratch must hold every touched list once plus the inserts
+  HIPCHK(launch_scratch_reset(e->actl.p, e->stream));
+  HIPCHK(launch_apply_merge(e->dev_graph(), b, e->scratch_cap, e->scratch.p, e->actl.p, e->stream));
+  HIPCHK(launch_apply_commit(e->dev_graph(), b, e->scratch.p, e->pool_cap, e->actl.p, e->dirty(), e->stream));
+  HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  for (int attempt = 0; e->h_actl.p->overflow && attempt < 4; attempt++) {
+    int rc = grow_pool(e, e->pool_cap / 2 + (1ull << 22));
+    if (rc) return rc;
+    // the merge itself overflowing scratch is impossible by sizing; only commits retry
+    HIPCHK(launch_apply_commit_failed(e->dev_graph(), b, e->scratch.p, e->pool_cap, e->actl.p, e->dirty(),
+                                      e->stream));
+    HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+  }
+  if (e->h_actl.p->overflow) return e->fail_msg(DSM_ENOMEM, "neighbour pool could not be grown");
+  return DSM_OK;
+}
+
+// Re-lays the pool with fresh slack (capacity deg*1.25+8 per vertex) into a
+// new allocation of the live size plus `extra` entries.
+int grow_pool(dsm_engine *e, unsigned long long extra) {
+  const uint32_t n = e->n;
+  std::vector<uint32_t> dg(n), st_new(n), cp_new(n);
+  HIPCHK(hipMemcpy(dg.data(), e->deg.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  unsigned long long total = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    cp_new[v] = dg[v] + dg[v] / 4 + 8;
+    st_new[v] = (uint32_t)total;
+    total += cp_new[v];
+  }
+  const unsigned long long ncap = total + extra;
+  if (ncap >= (1ull << 32)) return e->fail_msg(DSM_ELIMIT, "neighbour pool exceeds 32-bit offsets");
+  DevBuf<uint32_t> np, nst;
+  HIPCHK(np.alloc(ncap));
+  HIPCHK(nst.alloc(n));
+  HIPCHK(hipMemcpy(nst.p, st_new.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  HIPCHK(launch_pool_move(e->pool.p, np.p, e->start.p, nst.p, e->deg.p, n, e->stream));
+  HIPCHK(hipMemcpyAsync(e->cap.p, cp_new.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  std::swap(e->pool.p, np.p);
+  std::swap(e->pool.n, np.n);
+  std::swap(e->start.p, nst.p);
+  std::swap(e->start.n, nst.n);
+  e->pool_cap = ncap;
+  // scratch must hold one copy of every list
+  if (e->scratch_cap < ncap + (1ull << 20)) {
+    e->scratch_cap = ncap + (1ull << 20);
+    HIPCHK(e->scratch.alloc(e->scratch_cap));
+  }
+  ApplyCtl a = *e->h_actl.p;
+  a.pool_used = total;
+  a.overflow = 0;
+  HIPCHK(hipMemcpy(e->actl.p, &a, sizeof a, hipMemcpyHostToDevice));
+  *e->h_actl.p = a;
+  return DSM_OK;
+}
+
+int do_apply(dsm_engine *e, const dsm_change *ch, uint32_t k) {
+  HostDelta d;
+  host_delta(e, ch, k, d);
+  const uint32_t T = (uint32_t)d.tv.size();
+  // split into rounds of at most kMaxVertexChanges inserts and deletes per vertex
+  uint32_t rounds = 1;
+  for (uint32_t t = 0; t < T; t++) {
+    uint32_t m = std::max(d.ioff[t + 1] - d.ioff[t], d.doff[t + 1] - d.doff[t]);
+    rounds = std::max(rounds, (m + kMaxVertexChanges - 1) / kMaxVertexChanges);
+  }
+  for (uint32_t rd = 0; rd < rounds; rd++) {
+    std::vector<uint32_t> tv, ioff{0}, doff{0}, ins, del;
+    for (uint32_t t = 0; t < T; t++) {
+      uint32_t i0 = d.ioff[t] + rd * kMaxVertexChanges, i1 = std::min(d.ioff[t + 1], i0 + kMaxVertexChanges);
+      uint32_t d0 = d.doff[t] + rd * kMaxVertexChanges, d1 = std::min(d.doff[t + 1], d0 + kMaxVertexChanges);
+      if (i0 >= i1 && d0 >= d1 && rd > 0) continue;
+      tv.push_back(d.tv[t]);
+      if (i0 < i1) ins.insert(ins.end(), d.ins.begin() + i0, d.ins.begin() + i1);
+      if (d0 < d1) del.insert(del.end(), d.del.begin() + d0, d.del.begin() + d1);
+      ioff.push_back((uint32_t)ins.size());
+      doff.push_back((uint32_t)del.size());
+    }
+    int rc = apply_round(e, d, tv, ioff, doff, ins, del);
+    if (rc) return rc;
+  }
+  e->stats.touched_vertices = T;
+  e->stats.relocated_vertices = e->h_actl.p->relocated;
+  // relabels
+  BatchDev b{};
+  const uint32_t R = (uint32_t)d.rl_v.size();
+  if (R) {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(e->hbuf.ensure(2 * (size_t)R));
+    HIPCHK(e->bbuf.ensure(2 * (size_t)R));
+    std::copy(d.rl_v.begin(), d.rl_v.end(), e->hbuf.p);
+    std::copy(d.rl_l.begin(), d.rl_l.end(), e->hbuf.p + R);
+    HIPCHK(hipMemcpyAsync(e->bbuf.p, e->hbuf.p, 2 * (size_t)R * 4, hipMemcpyHostToDevice, e->stream));
+    b.nrelabel = R;
+    b.rl_v = e->bbuf.p;
+    b.rl_l = e->bbuf.p + R;
+    HIPCHK(launch_relabel(e->dev_graph(), b, e->tables(), e->dirty(), e->stream));
+  }
+  return DSM_OK;
+}
+
+int upload_records(dsm_engine *e, const std::vector<uint32_t> &s, const std::vector<uint32_t> &d,
+                   DevBuf<uint32_t> &dev) {
+  const size_t k = s.size();
+  PinnedBuf<uint32_t> &h = e->hbuf;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(dev.ensure(2 * k + 2));
+  HIPCHK(h.ensure(2 * k + 2));
+  std::copy(s.begin(), s.end(), h.p);
+  std::copy(d.begin(), d.end(), h.p + k);
+  HIPCHK(hipMemcpyAsync(dev.p, h.p, 2 * k * 4 + 4, hipMemcpyHostToDevice, e->stream));
+  return DSM_OK;
+}
+
+
+int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc *queries,
+                uint32_t num_queries) {
+  int rc = 0;
+  HIPCHK(hipSetDevice(e->opt.device));
+  HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIPCHK(hipDeviceGetAttribute(&e->num_cu, hipDeviceAttributeMultiprocessorCount, e->opt.device));
+  for (auto &x : e->ev) HIPCHK(hipEventCreate(&x));
+  if ((rc = build_queries(e, queries, num_queries))) return rc;
+  if ((rc = upload_graph(e, graph))) return rc;
+  if ((rc = upload_tables(e))) return rc;
+  uint32_t wpb = e->opt.waves_per_block;
+  if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) wpb = 4;
+  e->dfs_wpb = wpb;
+  int per_cu = dfs_max_blocks_per_cu(wpb);
+  if (e->opt.blocks_per_cu && (int)e->opt.blocks_per_cu < per_cu) per_cu = (int)e->opt.blocks_per_cu;
+  e->dfs_blocks = (uint32_t)(per_cu * e->num_cu);
+  HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return DSM_OK;
+}
+
+}  // namespace
+
+// =============================================================== C ABI
+
+extern "C" {
+
+void dsm_options_default(dsm_options *opt) {
+  std::memset(opt, 0, sizeof *opt);
+  opt->device = 0;
+  opt->relabel = 1;
+  opt->count_negatives = 1;
+  opt->waves_per_block = 4;
+}
+
+const char *dsm_last_error(const dsm_engine *e) {
+  if (!e) return g_create_error.c_str();
+  return e->err.c_str();
+}
+
+int dsm_create(const dsm_graph_desc *graph, const dsm_query_desc *queries, uint32_t num_queries,
+               const dsm_options *opt, dsm_engine **out) {
+  if (!out || !graph || !queries) {
+    g_create_error = "null argument";
+    return DSM_EINVAL;
+  }
+  *out = nullptr;
+  std::unique_ptr<dsm_engine> e(new dsm_engine());
+  if (opt) e->opt = *opt;
+  else dsm_options_default(&e->opt);
+  auto fail = [&](int rc) {
+    g_create_error = e->err;
+    return rc;
+  };
+  if (graph->num_vertices == 0) {
+    e->err = "empty data graph";
+    return fail(DSM_EINVAL);
+  }
+  e->n = graph->num_vertices;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    e->err = "no HIP device available";
+    return fail(DSM_EDEVICE);
+  }
+  if (e->opt.device < 0 || e->opt.device >= ndev) {
+    e->err = "device ordinal out of range";
+    return fail(DSM_EINVAL);
+  }
+  int rc = create_impl(e.get(), graph, queries, num_queries);
+  if (rc) return fail(rc);
+  *out = e.release();
+  return DSM_OK;
+}
+
+void dsm_destroy(dsm_engine *e) {
+  if (!e) return;
+  (void)hipSetDevice(e->opt.device);
+  (void)hipStreamSynchronize(e->stream);
+  delete e;
+}
+
+int dsm_apply_batch(dsm_engine *e, const dsm_change *changes, uint32_t k) {
+  if (!e || (!changes && k)) return DSM_EINVAL;
+  for (uint32_t i = 0; i < k; i++)
+    if (changes[i].src >= e->n || changes[i].dst >= e->n)
+      return e->fail_msg(DSM_EINVAL, "change %u: endpoint out of range (%u,%u) for %u vertices", i,
+                         changes[i].src, changes[i].dst, e->n);
+  HIPCHK(hipSetDevice(e->opt.device));
+  e->stats = dsm_stats{};
+  e->pos_s.clear();
+  e->pos_d.clear();
+  e->neg_s.clear();
+  e->neg_d.clear();
+  for (uint32_t i = 0; i < k; i++) {
+    if (changes[i].add) {
+      e->pos_s.push_back(changes[i].src);
+      e->pos_d.push_back(changes[i].dst);
+    } else {
+      e->neg_s.push_back(changes[i].src);
+      e->neg_d.push_back(changes[i].dst);
+    }
+  }
+  const uint32_t nq = (uint32_t)e->queries.size();
+  HIPCHK(launch_ctl_reset(nullptr, e->actl.p, e->tuple_count.p, e->counts.p, nq * 2, e->stream));
+  int rc;
+  // 1. negatives in the pre-batch graph
+  if (e->opt.count_negatives && !e->neg_s.empty()) {
+    const uint32_t nn = (uint32_t)e->neg_s.size();
+    if ((rc = ensure_seed_cap(e, nn))) return rc;
+    DevBuf<uint32_t> &rb = e->rec_neg;
+    if ((rc = upload_records(e, e->neg_s, e->neg_d, rb))) return rc;
+    if ((rc = run_round(e, rb.p, rb.p + nn, nn, 0, e->ev[6], e->ev[7], e->ev[8]))) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if ((rc = check_round(e, &e->stats.seeds_neg))) return rc;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[7], e->ev[8]));
+    e->stats.ms_dfs_neg = ms;
+  }
+  // 2-3. apply + relabel + filter
+  HIPCHK(hipEventRecord(e->ev[0], e->stream));
+  if ((rc = do_apply(e, changes, k))) return rc;
+  HIPCHK(hipEventRecord(e->ev[1], e->stream));
+  HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
+  HIPCHK(hipEventRecord(e->ev[2], e->stream));
+  HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+  e->stats.ms_apply = ms;
+  HIPCHK(hipEventElapsedTime(&ms, e->ev[1], e->ev[2]));
+  e->stats.ms_filter = ms;
+  e->stats.relabeled_vertices = e->h_actl.p->relabeled;
+  e->stats.dirty_groups = e->h_actl.p->ndirty;
+  e->stats.pool_entries_used = e->h_actl.p->pool_used;
+  e->stats.pool_entries_capacity = e->pool_cap;
+  e->batch_pending = true;
+  return DSM_OK;
+}
+
+int dsm_match_batch(dsm_engine *e, uint64_t *pos, uint64_t *neg) {
+  if (!e) return DSM_EINVAL;
+  HIPCHK(hipSetDevice(e->opt.device));
+  int rc;
+  const uint32_t nq = (uint32_t)e->queries.size();
+  if (e->batch_pending && !e->pos_s.empty()) {
+    const uint32_t np = (uint32_t)e->pos_s.size();
+    if ((rc = ensure_seed_cap(e, np))) return rc;
+    DevBuf<uint32_t> &rb = e->rec_pos;
+    if ((rc = upload_records(e, e->pos_s, e->pos_d, rb))) return rc;
+    if ((rc = run_round(e, rb.p, rb.p + np, np, 1, e->ev[3], e->ev[4], e->ev[5]))) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if ((rc = check_round(e, &e->stats.seeds_pos))) return rc;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[3], e->ev[4]));
+    e->stats.ms_seed = ms;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[4], e->ev[5]));
+    e->stats.ms_dfs_pos = ms;
+  }
+  e->batch_pending = false;
+  HIPCHK(hipMemcpyAsync(e->h_counts.p, e->counts.p, (size_t)nq * 2 * 8, hipMemcpyDeviceToHost, e->stream));
+  unsigned long long tc = 0;
+  HIPCHK(hipMemcpyAsync(&tc, e->tuple_count.p, 8, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->stats.tuples = tc;
+  for (uint32_t q = 0; q < nq; q++) {
+    if (pos) pos[q] = e->h_counts.p[2 * q + 1];
+    if (neg) neg[q] = e->h_counts.p[2 * q + 0];
+  }
+  return DSM_OK;
+}
+
+int dsm_process_batch(dsm_engine *e, const dsm_change *changes, uint32_t k, uint64_t *pos, uint64_t *neg) {
+  int rc = dsm_apply_batch(e, changes, k);
+  if (rc) return rc;
+  return dsm_match_batch(e, pos, neg);
+}
+
+int dsm_get_tuples(dsm_engine *e, uint32_t *out, uint64_t cap, uint64_t *num_out) {
+  if (!e || !num_out) return DSM_EINVAL;
+  const uint64_t found = e->stats.tuples;
+  *num_out = found;
+  const uint64_t have = std::min<uint64_t>(found, e->tuple_cap);
+  const uint64_t copy = std::min<uint64_t>(have, cap);
+  if (copy && out) {
+    HIPCHK(hipSetDevice(e->opt.device));
+    HIPCHK(hipMemcpy(out, e->tuples, copy * (kMaxQ + 2) * 4, hipMemcpyDefault));
+  }
+  if (found > e->tuple_cap) return e->fail_msg(DSM_ETRUNC, "%llu tuples found, capacity %llu",
+                                               (unsigned long long)found, (unsigned long long)e->tuple_cap);
+  return DSM_OK;
+}
+
+int dsm_get_stats(const dsm_engine *e, dsm_stats *out) {
+  if (!e || !out) return DSM_EINVAL;
+  *out = e->stats;
+  return DSM_OK;
+}
+
+int dsm_get_graph(dsm_engine *e, uint64_t *offsets, uint32_t *neighbours, uint64_t cap, uint32_t *labels,
+                  uint64_t *num_entries) {
+  if (!e || !offsets || !num_entries) return DSM_EINVAL;
+  HIPCHK(hipSetDevice(e->opt.device));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  const uint32_t n = e->n;
+  std::vector<uint32_t> st(n), dg(n);
+  std::vector<uint8_t> lb(n);
+  HIPCHK(hipMemcpy(st.data(), e->start.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(dg.data(), e->deg.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(lb.data(), e->lab.p, n, hipMemcpyDeviceToHost));
+  uint64_t o = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    offsets[v] = o;
+    o += dg[v];
+  }
+  offsets[n] = o;
+  *num_entries = o;
+  if (labels)
+    for (uint32_t v = 0; v < n; v++) labels[v] = lb[v];
+  if (o > cap || !neighbours) return o > cap ? e->fail_msg(DSM_ETRUNC, "neighbour buffer too small") : DSM_OK;
+  std::vector<uint32_t> hp((size_t)e->pool_cap);
+  HIPCHK(hipMemcpy(hp.data(), e->pool.p, (size_t)e->pool_cap * 4, hipMemcpyDeviceToHost));
+  for (uint32_t v = 0; v < n; v++) std::copy(hp.begin() + st[v], hp.begin() + st[v] + dg[v], neighbours + offsets[v]);
+  return DSM_OK;
+}
+
+int dsm_get_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words) {
+  if (!e || !words || q >= e->queries.size() || u >= e->queries[q].n) return DSM_EINVAL;
+  HIPCHK(hipSetDevice(e->opt.device));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  const uint32_t s = e->qsig[q][u];
+  HIPCHK(hipMemcpy(words, e->bm.p + (size_t)s * e->words, (size_t)e->words * 8, hipMemcpyDeviceToHost));
+  return DSM_OK;
+}
+
+uint32_t dsm_num_vertices(const dsm_engine *e) { return e ? e->n : 0; }
+uint32_t dsm_num_queries(const dsm_engine *e) { return e ? (uint32_t)e->queries.size() : 0; }
+
+}  // extern "C"

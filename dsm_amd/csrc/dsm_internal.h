@@ -1,0 +1,181 @@
+// Shared host/device definitions of the MI355X matching engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dsm {
+
+constexpr int kMaxQ = 16;       // query vertices per query (the reference's MAX_VQ, defs.h:18)
+constexpr int kWave = 64;       // CDNA wavefront
+constexpr int kMaxRel = 32;     // labels that can appear in an NLF requirement
+constexpr int kMaxVertexChanges = 1024;  // per vertex per apply round (host splits beyond)
+constexpr uint8_t kNoLabel = 255;  // device label of vertices whose label no query uses
+
+// Matching plan of one (query, ordered query edge (u0,u1)) pair: positions 0
+// and 1 hold the seeded edge, later positions follow a greedy order in which
+// every vertex has an earlier neighbour.
+struct alignas(16) Plan {
+  uint8_t n;             // query vertices
+  uint8_t pad0;
+  uint16_t query;        // query index
+  uint8_t qv[kMaxQ];     // query vertex at each position
+  uint16_t sig[kMaxQ];   // candidate bitmap of each position
+  uint16_t back[kMaxQ];  // earlier positions adjacent in the query
+  uint16_t inj[kMaxQ];   // earlier positions with the same label (injectivity)
+  uint32_t pad1[3];
+};
+static_assert(sizeof(Plan) == 128, "plan layout");
+
+// LDF/NLF requirement of one candidate bitmap: label, minimum degree and, for
+// each of up to 32 relevant labels (dense index), the minimum number of
+// neighbours with that label, one byte each (<= 15).
+struct alignas(8) SigReq {
+  uint32_t label;
+  uint32_t deg;
+  uint64_t nlc[4];
+};
+
+// A unit of DFS work: levels [0, level) are fixed in f; level `level` scans
+// source-list indices [lo, hi).
+struct alignas(16) Task {
+  uint32_t plan;
+  uint32_t level;
+  uint32_t lo, hi;
+  uint32_t f[kMaxQ];
+};
+constexpr int kTaskWords = sizeof(Task) / 4;  // 20
+
+// Device-resident control block of one seed + DFS round.
+struct alignas(64) DfsCtl {
+  uint32_t nseeds;        // seeds written by the seed kernel
+  uint32_t seed_head;     // next seed to claim
+  uint32_t q_head;        // next queue slot to claim
+  uint32_t q_tail;        // next queue slot to fill
+  int32_t work_left;      // seeds + donated tasks not yet finished
+  uint32_t idle;          // waves waiting for work
+  uint32_t donated;       // tasks donated this round
+  uint32_t seed_overflow; // seeds dropped for lack of capacity (must stay 0)
+  uint32_t pad[8];
+};
+
+// Status of the apply kernels of one batch.
+struct alignas(64) ApplyCtl {
+  unsigned long long scratch_used;
+  unsigned long long pool_used;     // bump pointer of the neighbour pool
+  uint32_t overflow;                // commits that found no room (host grows the pool)
+  uint32_t relocated;
+  uint32_t ndirty;                  // vertices on the dirty list
+  uint32_t nheavy;                  // dirty vertices re-filtered by a whole wave
+  uint32_t relabeled;
+  uint32_t pad[7];
+};
+
+struct DevGraph {
+  uint32_t n;
+  uint32_t *pool;     // neighbour lists, sorted ascending, with slack
+  uint32_t *start;    // first pool entry of each list
+  uint32_t *deg;      // list length
+  uint32_t *cap;      // slot capacity
+  uint8_t *lab;       // vertex label (kNoLabel when no query uses it)
+};
+
+struct FilterTables {
+  const SigReq *sigs;       // [nsig]
+  const uint32_t *sig_lo;   // [256] first sig of each label (sigs sorted by label)
+  const uint32_t *sig_hi;   // [256] one past the last
+  const uint8_t *lmap;      // [256] label -> dense relevant index or 0xFF
+  uint32_t nsig;
+  uint64_t *bm;             // [nsig][words]
+  uint32_t words;           // ceil(n/64)
+};
+
+// Batch description uploaded per apply round.
+struct BatchDev {
+  uint32_t ntouched;
+  const uint32_t *tv;       // touched vertices
+  const uint32_t *ioff;     // [ntouched+1] into ins
+  const uint32_t *doff;     // [ntouched+1] into del
+  const uint32_t *ins;
+  const uint32_t *del;
+  uint32_t *ndeg;           // [ntouched] new degree (merge -> commit)
+  unsigned long long *soff; // [ntouched] scratch offset
+  uint32_t *fail;           // [ntouched] commit found no room
+  uint32_t nrelabel;
+  const uint32_t *rl_v;
+  const uint32_t *rl_l;     // device label (already clamped)
+};
+
+struct DfsArgs {
+  DevGraph g;
+  const uint64_t *bm;
+  uint32_t words;
+  const Plan *plans;
+  const uint32_t *seed_plan;
+  const uint32_t *seed_s;
+  const uint32_t *seed_d;
+  Task *queue;
+  uint32_t *ready;
+  uint32_t qcap;
+  uint32_t epoch;
+  uint32_t donate_min;
+  uint32_t sign;                 // 1 positive, 0 negative
+  DfsCtl *ctl;
+  unsigned long long *counts;    // [nq][2]: index q*2+sign
+  uint32_t *tuples;              // [tuple_cap][kMaxQ+2] or null
+  unsigned long long tuple_cap;
+  unsigned long long *tuple_count;  // tuples found this batch
+};
+
+struct SeedArgs {
+  DevGraph g;
+  const uint64_t *bm;
+  uint32_t words;
+  const Plan *plans;
+  const uint32_t *plan_key;   // sorted (label0 << 8 | label1) of each plan
+  uint32_t nplans;
+  const uint32_t *rs;
+  const uint32_t *rd;
+  uint32_t nrec;
+  uint32_t *seed_plan;
+  uint32_t *seed_s;
+  uint32_t *seed_d;
+  uint32_t seed_cap;
+  DfsCtl *ctl;
+};
+
+struct DirtyArgs {
+  uint32_t *bits;      // [ceil(n/32)] dirty flags
+  uint32_t *list;      // dirty vertices
+  uint32_t *heavy;     // dirty vertices with a long list
+  uint32_t list_cap;
+  ApplyCtl *actl;
+};
+
+// ---- launchers (dsm_kernels.hip) -------------------------------------------
+hipError_t launch_apply_merge(const DevGraph &g, const BatchDev &b, unsigned long long scratch_cap,
+                              uint32_t *scratch, ApplyCtl *actl, hipStream_t s);
+hipError_t launch_apply_commit(const DevGraph &g, const BatchDev &b, const uint32_t *scratch,
+                               unsigned long long pool_cap, ApplyCtl *actl, const DirtyArgs &d,
+                               hipStream_t s);
+hipError_t launch_apply_commit_failed(const DevGraph &g, const BatchDev &b, const uint32_t *scratch,
+                                      unsigned long long pool_cap, ApplyCtl *actl, const DirtyArgs &d,
+                                      hipStream_t s);
+hipError_t launch_relabel(const DevGraph &g, const BatchDev &b, const FilterTables &ft,
+                          const DirtyArgs &d, hipStream_t s);
+hipError_t launch_filter_full(const DevGraph &g, const FilterTables &ft, hipStream_t s);
+hipError_t launch_filter_dirty(const DevGraph &g, const FilterTables &ft, const DirtyArgs &d,
+                               hipStream_t s);
+hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
+hipError_t launch_dfs(const DfsArgs &a, uint32_t blocks, uint32_t waves_per_block, hipStream_t s);
+// Resets the DFS control block (ctl) and/or the per-batch state (actl's
+// counters, the tuple counter and the nq*2 match counts); null pointers are skipped.
+hipError_t launch_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tuple_count,
+                            unsigned long long *counts, uint32_t ncounts, hipStream_t s);
+hipError_t launch_scratch_reset(ApplyCtl *actl, hipStream_t s);
+hipError_t launch_pool_move(const uint32_t *old_pool, uint32_t *new_pool, const uint32_t *old_start,
+                            const uint32_t *new_start, const uint32_t *deg, uint32_t n,
+                            hipStream_t s);
+int dfs_max_blocks_per_cu(uint32_t waves_per_block);
+
+}  // namespace dsm

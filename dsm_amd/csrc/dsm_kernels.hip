@@ -1,0 +1,910 @@
+// HIP kernels of the MI355X continuous subgraph matching engine (gfx950,
+// wave64). Integer work only; no MFMA.
+//
+//   k_apply_merge / k_apply_commit  in-place batch update of the slack CSR
+//                                   (the reference rebuilds the whole graph on
+//                                   the host, cpuGraph.cpp:521-607, and
+//                                   re-uploads it, io.cu:279-302)
+//   k_relabel                       label changes of the update rule
+//                                   (cpuGraph.cpp:564) + dirty marking
+//   k_filter_full / k_filter_light / k_filter_heavy
+//                                   LDF/NLF candidate bitmaps
+//                                   (filter.cu:394-566, compact filter.cu:1692)
+//   k_seed                          device edge seeding (main.cu:268-329;
+//                                   replaces the host ResTable and join.cu)
+//   k_dfs                           persistent wave64 DFS with per-wave LDS
+//                                   stacks and work donation (gpu_match.cu:
+//                                   96-876 restated for CDNA)
+#include "dsm_internal.h"
+
+namespace dsm {
+namespace {
+
+constexpr int kApplyWaves = 4;
+constexpr int kLight = 64;          // dirty vertices up to this degree: one lane each
+constexpr uint32_t kDonateEvery = 8;  // chunks between idle checks
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ uint32_t bcast(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t bcast64(uint64_t x) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_relaxed(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_acquire(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_release(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool cas_u32(uint32_t *p, uint32_t expect, uint32_t desire) {
+  return __hip_atomic_compare_exchange_strong(p, &expect, desire, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// # elements < x in sorted a[0, n)
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t *a, uint32_t n, uint32_t x) {
+  uint32_t lo = 0;
+  while (n > 0) {
+    uint32_t half = n >> 1;
+    if (a[lo + half] < x) {
+      lo += half + 1;
+      n -= half + 1;
+    } else {
+      n = half;
+    }
+  }
+  return lo;
+}
+
+__device__ __forceinline__ bool contains(const uint32_t *a, uint32_t n, uint32_t x) {
+  if (n == 0) return false;
+  uint32_t lo = 0;
+  while (n > 1) {
+    uint32_t half = n >> 1;
+    lo = (a[lo + half] <= x) ? lo + half : lo;
+    n -= half;
+  }
+  return a[lo] == x;
+}
+
+__device__ __forceinline__ bool bit_of(const uint64_t *bm, uint32_t words, uint32_t sig, uint32_t v) {
+  return (bm[(size_t)sig * words + (v >> 6)] >> (v & 63)) & 1ull;
+}
+
+// all 8 byte fields of x >= those of y (fields hold 0..15)
+__device__ __forceinline__ bool ge_bytes(uint64_t x, uint64_t y) {
+  constexpr uint64_t H = 0x8080808080808080ull;
+  return (((x | H) - y) & H) == H;
+}
+
+__device__ __forceinline__ void mark_dirty(const DirtyArgs &d, uint32_t x) {
+  const uint32_t bit = 1u << (x & 31);
+  const uint32_t old = atomicOr(&d.bits[x >> 5], bit);
+  if (!(old & bit)) {
+    const uint32_t p = atomicAdd(&d.actl->ndirty, 1u);
+    if (p < d.list_cap) d.list[p] = x;
+  }
+}
+
+// ------------------------------------------------------------------ apply
+
+// One wave per touched vertex: new list = (A \ D) u I written to scratch.
+// I and D are sorted and hold at most kMaxVertexChanges entries each; every
+// key has a single final operation, so I and D are disjoint.
+__global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, BatchDev b,
+                                                                  unsigned long long scratch_cap,
+                                                                  uint32_t *scratch, ApplyCtl *actl) {
+  __shared__ uint16_t s_icum[kApplyWaves][kMaxVertexChanges + 1];
+  __shared__ uint16_t s_dcum[kApplyWaves][kMaxVertexChanges + 1];
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  uint16_t *icum = s_icum[wv];
+  uint16_t *dcum = s_dcum[wv];
+  for (uint32_t t = blockIdx.x * kApplyWaves + wv; t < b.ntouched; t += gridDim.x * kApplyWaves) {
+    const uint32_t x = b.tv[t];
+    const uint32_t *A = g.pool + g.start[x];
+    const uint32_t na = g.deg[x];
+    const uint32_t i0 = b.ioff[t], ni = b.ioff[t + 1] - i0;
+    const uint32_t d0 = b.doff[t], nd = b.doff[t + 1] - d0;
+    const uint32_t *I = b.ins + i0;
+    const uint32_t *D = b.del + d0;
+    // exclusive prefix of "insert is new" and "delete hits"
+    uint32_t run = 0;
+    for (uint32_t c = 0; c < ni; c += 64) {
+      const uint32_t j = c + lane;
+      const bool fl = j < ni && !contains(A, na, I[j]);
+      const uint64_t m = ballot(fl);
+      if (j < ni) icum[j] = (uint16_t)(run + mbcnt(m));
+      run += __popcll(m);
+    }
+    const uint32_t nnew = run;
+    run = 0;
+    for (uint32_t c = 0; c < nd; c += 64) {
+      const uint32_t j = c + lane;
+      const bool fl = j < nd && contains(A, na, D[j]);
+      const uint64_t m = ballot(fl);
+      if (j < nd) dcum[j] = (uint16_t)(run + mbcnt(m));
+      run += __popcll(m);
+    }
+    const uint32_t nhit = run;
+    if (lane == 0) {
+      icum[ni] = (uint16_t)nnew;
+      dcum[nd] = (uint16_t)nhit;
+    }
+    wave_sync();
+    const uint32_t newdeg = na - nhit + nnew;
+    unsigned long long so = 0;
+    if (lane == 0) so = atomicAdd(&actl->scratch_used, (unsigned long long)(na + nnew));
+    so = bcast64(so);
+    if (so + na + nnew > scratch_cap) {  // sized by the host so that this cannot happen
+      if (lane == 0) {
+        b.ndeg[t] = na;
+        b.soff[t] = ~0ull;
+        atomicAdd(&actl->overflow, 1u);
+      }
+      continue;
+    }
+    uint32_t *out = scratch + so;
+    for (uint32_t j = lane; j < na; j += 64) {
+      const uint32_t a = A[j];
+      const uint32_t p = lower_bound(D, nd, a);
+      if (p < nd && D[p] == a) continue;
+      const uint32_t q = lower_bound(I, ni, a);
+      out[j - dcum[p] + icum[q]] = a;
+    }
+    for (uint32_t j = lane; j < ni; j += 64) {
+      if (icum[j + 1] == icum[j]) continue;  // already present
+      const uint32_t i = I[j];
+      const uint32_t r = lower_bound(A, na, i);
+      const uint32_t p = lower_bound(D, nd, i);
+      out[r - dcum[p] + icum[j]] = i;
+    }
+    if (lane == 0) {
+      b.ndeg[t] = newdeg;
+      b.soff[t] = so;
+    }
+    wave_sync();
+  }
+}
+
+// One wave per touched vertex: copy the merged list back into its slot, or
+// into a new, larger slot taken from the pool's bump pointer. A vertex that
+// finds no room keeps its old list and is flagged for a retry after the host
+// grows the pool. With only_failed, only flagged vertices are processed.
+__global__ __launch_bounds__(kApplyWaves * 64) void k_apply_commit(DevGraph g, BatchDev b,
+                                                                   const uint32_t *scratch,
+                                                                   unsigned long long pool_cap,
+                                                                   ApplyCtl *actl, DirtyArgs d,
+                                                                   int only_failed) {
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  for (uint32_t t = blockIdx.x * kApplyWaves + wv; t < b.ntouched; t += gridDim.x * kApplyWaves) {
+    if (only_failed && b.fail[t] == 0) continue;
+    const uint32_t x = b.tv[t];
+    const uint32_t nd = b.ndeg[t];
+    const unsigned long long so = b.soff[t];
+    if (so == ~0ull) continue;
+    uint32_t dst = g.start[x];
+    if (nd > g.cap[x]) {
+      const uint32_t newcap = nd + nd / 4 + 8;
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&actl->pool_used, (unsigned long long)newcap);
+      base = bcast64(base);
+      if (base + newcap > pool_cap) {
+        if (lane == 0) {
+          b.fail[t] = 1;
+          atomicAdd(&actl->overflow, 1u);
+        }
+        continue;
+      }
+      dst = (uint32_t)base;
+      if (lane == 0) {
+        g.start[x] = dst;
+        g.cap[x] = newcap;
+        atomicAdd(&actl->relocated, 1u);
+      }
+    }
+    for (uint32_t j = lane; j < nd; j += 64) g.pool[dst + j] = scratch[so + j];
+    if (lane == 0) {
+      g.deg[x] = nd;
+      b.fail[t] = 0;
+      mark_dirty(d, x);
+    }
+  }
+}
+
+// One wave per relabel entry: set the label; when it changed, clear the
+// vertex's bits in every bitmap of its old label and mark it and all its
+// (post-update) neighbours dirty, since their neighbour-label counts changed.
+__global__ __launch_bounds__(256) void k_relabel(DevGraph g, BatchDev b, FilterTables ft, DirtyArgs d) {
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  for (uint32_t r = blockIdx.x * 4 + wv; r < b.nrelabel; r += gridDim.x * 4) {
+    const uint32_t x = b.rl_v[r];
+    const uint32_t L = b.rl_l[r];
+    const uint32_t old = g.lab[x];
+    if (old == L) continue;
+    if (lane == 0) {
+      g.lab[x] = (uint8_t)L;
+      atomicAdd(&d.actl->relabeled, 1u);
+      mark_dirty(d, x);
+    }
+    if (old != kNoLabel) {
+      const uint64_t clr = ~(1ull << (x & 63));
+      for (uint32_t s = ft.sig_lo[old] + lane; s < ft.sig_hi[old]; s += 64)
+        atomicAnd((unsigned long long *)&ft.bm[(size_t)s * ft.words + (x >> 6)], (unsigned long long)clr);
+    }
+    const uint32_t *A = g.pool + g.start[x];
+    const uint32_t na = g.deg[x];
+    for (uint32_t j = lane; j < na; j += 64) mark_dirty(d, A[j]);
+  }
+}
+
+// ----------------------------------------------------------------- filter
+
+__device__ __forceinline__ bool sig_pass(const SigReq &r, uint32_t deg, const uint64_t c[4]) {
+  return deg >= r.deg && ge_bytes(c[0], r.nlc[0]) && ge_bytes(c[1], r.nlc[1]) &&
+         ge_bytes(c[2], r.nlc[2]) && ge_bytes(c[3], r.nlc[3]);
+}
+
+// Whole-graph pass, one wave per 64-vertex group: the group's adjacency lists
+// are scanned as one logical array (lanes balanced across vertices), label
+// counts accumulate in LDS, then every bitmap word of the group is written
+// with one ballot.
+__global__ __launch_bounds__(256) void k_filter_full(DevGraph g, FilterTables ft) {
+  __shared__ uint32_t s_cnt[4][64][kMaxRel + 1];
+  __shared__ uint32_t s_pre[4][65];
+  __shared__ uint32_t s_st[4][64];
+  __shared__ uint8_t s_lmap[256];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = ft.lmap[i];
+  __syncthreads();
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t ngroups = (g.n + 63) / 64;
+  for (uint32_t grp = blockIdx.x * 4 + wv; grp < ngroups; grp += gridDim.x * 4) {
+    const uint32_t v = grp * 64 + lane;
+    const bool valid = v < g.n;
+    const uint32_t L = valid ? g.lab[v] : kNoLabel;
+    const bool has = L != kNoLabel && ft.sig_lo[L] < ft.sig_hi[L];
+    const uint32_t dv = valid ? g.deg[v] : 0;
+    const uint32_t d = has ? dv : 0;
+    s_st[wv][lane] = valid ? g.start[v] : 0;
+    for (int l = 0; l <= kMaxRel; l++) s_cnt[wv][lane][l] = 0;
+    // inclusive scan of d
+    uint32_t incl = d;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off);
+      if (lane >= (uint32_t)off) incl += y;
+    }
+    s_pre[wv][lane + 1] = incl;
+    if (lane == 0) s_pre[wv][0] = 0;
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    wave_sync();
+    for (uint32_t e = lane; e < total; e += 64) {
+      // owner: last k with s_pre[k] <= e
+      uint32_t lo = 0, n = 64;
+      while (n > 1) {
+        const uint32_t half = n >> 1;
+        lo = (s_pre[wv][lo + half] <= e) ? lo + half : lo;
+        n -= half;
+      }
+      const uint32_t w = g.pool[s_st[wv][lo] + (e - s_pre[wv][lo])];
+      const uint32_t li = s_lmap[g.lab[w]];
+      if (li != 0xFF) atomicAdd(&s_cnt[wv][lo][li], 1u);
+    }
+    wave_sync();
+    uint64_t c[4] = {0, 0, 0, 0};
+    for (int l = 0; l < kMaxRel; l++) {
+      const uint64_t x = s_cnt[wv][lane][l] < 15 ? s_cnt[wv][lane][l] : 15;
+      c[l >> 3] |= x << ((l & 7) * 8);
+    }
+    for (uint32_t s = 0; s < ft.nsig; s++) {
+      const SigReq r = ft.sigs[s];
+      const bool pass = has && L == r.label && sig_pass(r, dv, c);
+      const uint64_t word = ballot(pass);
+      if (lane == 0) ft.bm[(size_t)s * ft.words + grp] = word;
+    }
+    wave_sync();
+  }
+}
+
+__device__ __forceinline__ void set_bit(uint64_t *w, uint32_t v, bool on) {
+  const uint64_t bit = 1ull << (v & 63);
+  const bool cur = (*w & bit) != 0;  // only this thread changes this bit in this kernel
+  if (cur == on) return;
+  if (on) atomicOr((unsigned long long *)w, (unsigned long long)bit);
+  else atomicAnd((unsigned long long *)w, (unsigned long long)~bit);
+}
+
+// Dirty vertices, one lane each; lists longer than kLight go to the heavy list.
+__global__ __launch_bounds__(256) void k_filter_light(DevGraph g, FilterTables ft, DirtyArgs d) {
+  __shared__ uint8_t s_cnt[256][kMaxRel];
+  __shared__ uint8_t s_lmap[256];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = ft.lmap[i];
+  __syncthreads();
+  const uint32_t nd = d.actl->ndirty < d.list_cap ? d.actl->ndirty : d.list_cap;
+  uint8_t *cnt = s_cnt[threadIdx.x];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += gridDim.x * blockDim.x) {
+    const uint32_t v = d.list[i];
+    atomicAnd(&d.bits[v >> 5], ~(1u << (v & 31)));
+    const uint32_t L = g.lab[v];
+    if (L == kNoLabel || ft.sig_lo[L] == ft.sig_hi[L]) continue;
+    const uint32_t dv = g.deg[v];
+    if (dv > (uint32_t)kLight) {
+      const uint32_t p = atomicAdd(&d.actl->nheavy, 1u);
+      if (p < d.list_cap) d.heavy[p] = v;
+      continue;
+    }
+    for (int l = 0; l < kMaxRel; l++) cnt[l] = 0;
+    const uint32_t *A = g.pool + g.start[v];
+    for (uint32_t j = 0; j < dv; j++) {
+      const uint32_t li = s_lmap[g.lab[A[j]]];
+      if (li != 0xFF && cnt[li] < 15) cnt[li]++;
+    }
+    uint64_t c[4];
+    for (int k = 0; k < 4; k++) {
+      uint64_t x = 0;
+      for (int b = 0; b < 8; b++) x |= (uint64_t)cnt[k * 8 + b] << (b * 8);
+      c[k] = x;
+    }
+    for (uint32_t s = ft.sig_lo[L]; s < ft.sig_hi[L]; s++)
+      set_bit(&ft.bm[(size_t)s * ft.words + (v >> 6)], v, sig_pass(ft.sigs[s], dv, c));
+  }
+}
+
+// Dirty vertices with long lists, one wave each; lanes then test bitmaps in parallel.
+__global__ __launch_bounds__(256) void k_filter_heavy(DevGraph g, FilterTables ft, DirtyArgs d) {
+  __shared__ uint32_t s_cnt[4][kMaxRel];
+  __shared__ uint64_t s_pack[4][4];
+  __shared__ uint8_t s_lmap[256];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = ft.lmap[i];
+  __syncthreads();
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t nh = d.actl->nheavy < d.list_cap ? d.actl->nheavy : d.list_cap;
+  for (uint32_t h = blockIdx.x * 4 + wv; h < nh; h += gridDim.x * 4) {
+    const uint32_t v = d.heavy[h];
+    const uint32_t L = g.lab[v];
+    const uint32_t dv = g.deg[v];
+    const uint32_t *A = g.pool + g.start[v];
+    if (lane < (uint32_t)kMaxRel) s_cnt[wv][lane] = 0;
+    wave_sync();
+    for (uint32_t j = lane; j < dv; j += 64) {
+      const uint32_t li = s_lmap[g.lab[A[j]]];
+      if (li != 0xFF) atomicAdd(&s_cnt[wv][li], 1u);
+    }
+    wave_sync();
+    if (lane < 4) {
+      uint64_t x = 0;
+      for (int b = 0; b < 8; b++) {
+        const uint32_t y = s_cnt[wv][lane * 8 + b];
+        x |= (uint64_t)(y < 15 ? y : 15) << (b * 8);
+      }
+      s_pack[wv][lane] = x;
+    }
+    wave_sync();
+    const uint64_t c[4] = {s_pack[wv][0], s_pack[wv][1], s_pack[wv][2], s_pack[wv][3]};
+    for (uint32_t s = ft.sig_lo[L] + lane; s < ft.sig_hi[L]; s += 64)
+      set_bit(&ft.bm[(size_t)s * ft.words + (v >> 6)], v, sig_pass(ft.sigs[s], dv, c));
+    wave_sync();
+  }
+}
+
+// ------------------------------------------------------------------- seed
+
+// One thread per record (s,d): every plan whose seeded query edge (u0,u1) has
+// s in C(u0) and d in C(u1) becomes a DFS root, provided (s,d) is an edge of
+// the graph being matched.
+__global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nrec; r += gridDim.x * blockDim.x) {
+    const uint32_t s = a.rs[r], d = a.rd[r];
+    if (s == d || s >= a.g.n || d >= a.g.n) continue;
+    const uint32_t ls = a.g.lab[s], ld = a.g.lab[d];
+    if (ls == kNoLabel || ld == kNoLabel) continue;
+    const uint32_t key = (ls << 8) | ld;
+    const uint32_t p0 = lower_bound(a.plan_key, a.nplans, key);
+    const uint32_t p1 = lower_bound(a.plan_key, a.nplans, key + 1);
+    if (p0 == p1) continue;
+    uint32_t cnt = 0;
+    for (uint32_t p = p0; p < p1; p++) {
+      const Plan &pl = a.plans[p];
+      cnt += bit_of(a.bm, a.words, pl.sig[0], s) && bit_of(a.bm, a.words, pl.sig[1], d);
+    }
+    if (cnt == 0) continue;
+    // d in N(s): the direction the DFS also tests (w in the list of an
+    // earlier image); the two coincide whenever the graph is symmetric
+    if (!contains(a.g.pool + a.g.start[s], a.g.deg[s], d)) continue;
+    const uint32_t base = atomicAdd(&a.ctl->nseeds, cnt);
+    if (base + cnt > a.seed_cap) {
+      atomicAdd(&a.ctl->seed_overflow, cnt);
+      continue;
+    }
+    atomicAdd(&a.ctl->work_left, (int)cnt);
+    uint32_t o = base;
+    for (uint32_t p = p0; p < p1; p++) {
+      const Plan &pl = a.plans[p];
+      if (bit_of(a.bm, a.words, pl.sig[0], s) && bit_of(a.bm, a.words, pl.sig[1], d)) {
+        a.seed_plan[o] = p;
+        a.seed_s[o] = s;
+        a.seed_d[o] = d;
+        o++;
+      }
+    }
+  }
+}
+
+__global__ void k_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tuple_count,
+                            unsigned long long *counts, uint32_t ncounts) {
+  for (uint32_t i = threadIdx.x; counts && i < ncounts; i += blockDim.x) counts[i] = 0;
+  if (threadIdx.x == 0) {
+    if (tuple_count) *tuple_count = 0;
+    if (ctl) {
+      ctl->nseeds = 0;
+      ctl->seed_head = 0;
+      ctl->q_head = 0;
+      ctl->q_tail = 0;
+      ctl->work_left = 0;
+      ctl->idle = 0;
+      ctl->donated = 0;
+      ctl->seed_overflow = 0;
+    }
+    if (actl) {
+      actl->scratch_used = 0;
+      actl->overflow = 0;
+      actl->relocated = 0;
+      actl->ndirty = 0;
+      actl->nheavy = 0;
+      actl->relabeled = 0;
+    }
+  }
+}
+
+__global__ void k_scratch_reset(ApplyCtl *actl) {
+  if (threadIdx.x == 0) actl->scratch_used = 0;
+}
+
+// -------------------------------------------------------------------- DFS
+
+struct WaveLds {
+  uint32_t cand[kMaxQ][64];  // current candidate chunk of each level
+  uint32_t f[kMaxQ];         // data vertex of each position
+  uint32_t fs[kMaxQ];        // its list start
+  uint32_t fd[kMaxQ];        // its degree
+  uint32_t srcj[kMaxQ];      // backward position whose list drives the scan
+  uint32_t pos[kMaxQ];       // next source index
+  uint32_t end[kMaxQ];       // one past the last source index
+  uint32_t cn[kMaxQ];        // chunk size
+  uint32_t ci[kMaxQ];        // next chunk entry
+  uint16_t sig[kMaxQ];
+  uint16_t back[kMaxQ];
+  uint16_t inj[kMaxQ];
+  uint8_t qv[kMaxQ];
+  uint32_t n, query, plan;
+};
+
+__device__ __forceinline__ void assign(WaveLds &S, const DevGraph &g, uint32_t l, uint32_t w) {
+  if (lane_id() == 0) {
+    S.f[l] = w;
+    S.fs[l] = g.start[w];
+    S.fd[l] = g.deg[w];
+  }
+  wave_sync();
+}
+
+__device__ __forceinline__ void init_level(WaveLds &S, uint32_t l, uint32_t lo, uint32_t hi) {
+  const uint32_t bm = S.back[l];
+  uint32_t best = bm ? 0xFFFFFFFFu : 0u, bj = 0;  // no backward neighbour: empty scan
+  for (uint32_t j = 0; j < l; j++)
+    if ((bm >> j & 1) && S.fd[j] < best) {
+      best = S.fd[j];
+      bj = j;
+    }
+  wave_sync();
+  if (lane_id() == 0) {
+    S.srcj[l] = bj;
+    S.pos[l] = lo;
+    S.end[l] = hi < best ? hi : best;
+    S.cn[l] = 0;
+    S.ci[l] = 0;
+  }
+  wave_sync();
+}
+
+// Is w a valid image for position l: candidate bit, injectivity against
+// earlier positions of the same label, adjacency to every backward neighbour
+// other than the one whose list produced w.
+__device__ __forceinline__ bool check(const WaveLds &S, const DfsArgs &a, uint32_t l, uint32_t w) {
+  if (!bit_of(a.bm, a.words, S.sig[l], w)) return false;
+  uint32_t inj = S.inj[l];
+  while (inj) {
+    const uint32_t j = __builtin_ctz(inj);
+    inj &= inj - 1;
+    if (S.f[j] == w) return false;
+  }
+  uint32_t oth = S.back[l] & ~(1u << S.srcj[l]);
+  while (oth) {
+    const uint32_t j = __builtin_ctz(oth);
+    oth &= oth - 1;
+    if (!contains(a.g.pool + S.fs[j], S.fd[j], w)) return false;
+  }
+  return true;
+}
+
+__device__ void emit_tuples(const WaveLds &S, const DfsArgs &a, uint32_t l, uint32_t w, bool ok) {
+  const uint64_t m = ballot(ok);
+  if (m == 0) return;
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(a.tuple_count, (unsigned long long)__popcll(m));
+  base = bcast64(base);
+  if (!ok) return;
+  const unsigned long long idx = base + mbcnt(m);
+  if (idx >= a.tuple_cap) return;
+  uint32_t *t = a.tuples + idx * (kMaxQ + 2);
+  t[0] = S.query;
+  t[1] = a.sign;
+  for (uint32_t k = S.n; k < (uint32_t)kMaxQ; k++) t[2 + k] = 0xFFFFFFFFu;
+  for (uint32_t j = 0; j < l; j++) t[2 + S.qv[j]] = S.f[j];
+  t[2 + S.qv[l]] = w;
+}
+
+// Hands half of the shallowest large-enough remaining source range to the
+// queue when some wave is idle.
+__device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l) {
+  uint32_t idle = 0;
+  if (lane_id() == 0) idle = ld_relaxed(&a.ctl->idle);
+  idle = bcast(idle);
+  if (idle == 0) return;
+  uint32_t lv = 0xFFFFFFFFu;
+  for (uint32_t k = L0; k <= l; k++) {
+    const uint32_t p = S.pos[k], e = S.end[k];
+    if (e > p && e - p >= a.donate_min) {
+      lv = k;
+      break;
+    }
+  }
+  if (lv == 0xFFFFFFFFu) return;
+  const uint32_t p = S.pos[lv], e = S.end[lv];
+  const uint32_t mid = p + (e - p) / 2;
+  uint32_t slot = 0, ok = 0;
+  if (lane_id() == 0) {
+    const uint32_t t = ld_relaxed(&a.ctl->q_tail);
+    if (t < a.qcap && cas_u32(&a.ctl->q_tail, t, t + 1)) {
+      ok = 1;
+      slot = t;
+      atomicAdd(&a.ctl->work_left, 1);
+      atomicAdd(&a.ctl->donated, 1u);
+    }
+  }
+  ok = bcast(ok);
+  if (!ok) return;
+  slot = bcast(slot);
+  const uint32_t lane = lane_id();
+  uint32_t word = 0;
+  if (lane == 0) word = S.plan;
+  else if (lane == 1) word = lv;
+  else if (lane == 2) word = mid;
+  else if (lane == 3) word = e;
+  else if (lane < (uint32_t)kTaskWords) word = S.f[lane - 4];
+  if (lane < (uint32_t)kTaskWords) reinterpret_cast<uint32_t *>(&a.queue[slot])[lane] = word;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (lane == 0) {
+    st_release(&a.ready[slot], a.epoch);
+    S.end[lv] = mid;
+  }
+  wave_sync();
+}
+
+template <int WPB, bool TUP>
+__global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
+  __shared__ WaveLds s_all[WPB];
+  WaveLds &S = s_all[threadIdx.x >> 6];
+  const uint32_t lane = lane_id();
+  if (lane == 0) S.plan = 0xFFFFFFFFu;
+  wave_sync();
+  bool idle = false;
+  // a seed table overflow leaves unwritten slots: match nothing, the host
+  // reports the error (the table is sized so that it cannot happen)
+  if (a.ctl->seed_overflow) return;
+  const uint32_t nseeds = a.ctl->nseeds;
+  for (;;) {
+    uint32_t got = 0, slot = 0;
+    if (lane == 0) {
+      uint32_t h = ld_relaxed(&a.ctl->q_head);
+      while (h < ld_relaxed(&a.ctl->q_tail)) {
+        if (cas_u32(&a.ctl->q_head, h, h + 1)) {
+          got = 1;
+          slot = h;
+          break;
+        }
+        h = ld_relaxed(&a.ctl->q_head);
+      }
+      if (!got && ld_relaxed(&a.ctl->seed_head) < nseeds) {
+        const uint32_t i = atomicAdd(&a.ctl->seed_head, 1u);
+        if (i < nseeds) {
+          got = 2;
+          slot = i;
+        }
+      }
+    }
+    got = bcast(got);
+    slot = bcast(slot);
+    if (got == 0) {
+      int32_t left = 0;
+      if (lane == 0) left = ld_relaxed(&a.ctl->work_left);
+      left = (int32_t)bcast((uint32_t)left);
+      if (left <= 0) break;
+      if (!idle) {
+        if (lane == 0) atomicAdd(&a.ctl->idle, 1u);
+        idle = true;
+      }
+      __builtin_amdgcn_s_sleep(16);
+      continue;
+    }
+    if (idle) {
+      if (lane == 0) atomicSub(&a.ctl->idle, 1u);
+      idle = false;
+    }
+    // ---- load the task
+    uint32_t plan, L0, lo, hi, fw = 0;
+    if (got == 1) {
+      if (lane == 0)
+        while (ld_acquire(&a.ready[slot]) != a.epoch) __builtin_amdgcn_s_sleep(1);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      const uint32_t word =
+          lane < (uint32_t)kTaskWords ? reinterpret_cast<const uint32_t *>(&a.queue[slot])[lane] : 0u;
+      plan = __builtin_amdgcn_readlane(word, 0);
+      L0 = __builtin_amdgcn_readlane(word, 1);
+      lo = __builtin_amdgcn_readlane(word, 2);
+      hi = __builtin_amdgcn_readlane(word, 3);
+      fw = __shfl(word, (int)((lane + 4) & 63));
+    } else {
+      plan = a.seed_plan[slot];
+      L0 = 2;
+      lo = 0;
+      hi = 0xFFFFFFFFu;
+      fw = lane == 0 ? a.seed_s[slot] : a.seed_d[slot];
+    }
+    if (bcast(S.plan) != plan) {
+      const Plan &P = a.plans[plan];
+      if (lane < (uint32_t)kMaxQ) {
+        S.sig[lane] = P.sig[lane];
+        S.back[lane] = P.back[lane];
+        S.inj[lane] = P.inj[lane];
+        S.qv[lane] = P.qv[lane];
+      }
+      if (lane == 0) {
+        S.n = P.n;
+        S.query = P.query;
+        S.plan = plan;
+      }
+    }
+    if (lane < L0) {
+      S.f[lane] = fw;
+      S.fs[lane] = a.g.start[fw];
+      S.fd[lane] = a.g.deg[fw];
+    }
+    wave_sync();
+    const uint32_t n = bcast(S.n);
+    uint64_t lcount = 0;
+    if (L0 >= n) {
+      // complete embedding (two-vertex query seeded by its only edge)
+      if (lane == 0) lcount = 1;
+      if (TUP) emit_tuples(S, a, n - 1, S.f[n - 1], lane == 0);
+    } else {
+      init_level(S, L0, lo, hi);
+      uint32_t l = L0;
+      uint32_t tick = 0;
+      for (;;) {
+        if (l == n - 1) {
+          // leaf: count every valid image in the remaining source range
+          for (;;) {
+            if (++tick >= kDonateEvery) {
+              tick = 0;
+              try_donate(S, a, L0, l);
+            }
+            const uint32_t p = bcast(S.pos[l]), e = bcast(S.end[l]);
+            if (p >= e) break;
+            const uint32_t i = p + lane;
+            bool ok = false;
+            uint32_t w = 0;
+            if (i < e) {
+              w = a.g.pool[S.fs[S.srcj[l]] + i];
+              ok = check(S, a, l, w);
+            }
+            lcount += ok ? 1u : 0u;
+            if (TUP) emit_tuples(S, a, l, w, ok);
+            wave_sync();
+            if (lane == 0) S.pos[l] = p + 64;
+            wave_sync();
+          }
+          if (l == L0) break;
+          l--;
+          continue;
+        }
+        const uint32_t ci = bcast(S.ci[l]), cn = bcast(S.cn[l]);
+        if (ci < cn) {
+          const uint32_t w = bcast(S.cand[l][ci]);
+          wave_sync();
+          if (lane == 0) S.ci[l] = ci + 1;
+          assign(S, a.g, l, w);
+          l++;
+          init_level(S, l, 0, 0xFFFFFFFFu);
+          continue;
+        }
+        if (++tick >= kDonateEvery) {
+          tick = 0;
+          try_donate(S, a, L0, l);
+        }
+        const uint32_t p = bcast(S.pos[l]), e = bcast(S.end[l]);
+        if (p < e) {
+          const uint32_t i = p + lane;
+          bool ok = false;
+          uint32_t w = 0;
+          if (i < e) {
+            w = a.g.pool[S.fs[S.srcj[l]] + i];
+            ok = check(S, a, l, w);
+          }
+          const uint64_t m = ballot(ok);
+          wave_sync();
+          if (ok) S.cand[l][mbcnt(m)] = w;
+          if (lane == 0) {
+            S.cn[l] = __popcll(m);
+            S.ci[l] = 0;
+            S.pos[l] = p + 64;
+          }
+          wave_sync();
+          continue;
+        }
+        if (l == L0) break;
+        l--;
+      }
+    }
+    // wave sum of the lane counts
+    uint64_t tot = lcount;
+    for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off);
+    if (lane == 0) {
+      if (tot) atomicAdd(&a.counts[(size_t)bcast(S.query) * 2 + a.sign], (unsigned long long)tot);
+      atomicSub(&a.ctl->work_left, 1);
+    }
+  }
+}
+
+// Copies every vertex's live list to its new slot (pool growth/compaction).
+__global__ __launch_bounds__(256) void k_pool_move(const uint32_t *old_pool, uint32_t *new_pool,
+                                                   const uint32_t *old_start, const uint32_t *new_start,
+                                                   const uint32_t *deg, uint32_t n) {
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  for (uint32_t v = blockIdx.x * 4 + wv; v < n; v += gridDim.x * 4) {
+    const uint32_t d = deg[v];
+    const uint32_t *src = old_pool + old_start[v];
+    uint32_t *dst = new_pool + new_start[v];
+    for (uint32_t j = lane; j < d; j += 64) dst[j] = src[j];
+  }
+}
+
+template <int WPB>
+int max_blocks() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_dfs<WPB, false>, WPB * 64, 0) != hipSuccess) return 1;
+  return nb > 0 ? nb : 1;
+}
+
+}  // namespace
+
+hipError_t launch_apply_merge(const DevGraph &g, const BatchDev &b, unsigned long long scratch_cap,
+                              uint32_t *scratch, ApplyCtl *actl, hipStream_t s) {
+  if (b.ntouched == 0) return hipSuccess;
+  const uint32_t blocks = (b.ntouched + kApplyWaves - 1) / kApplyWaves;
+  hipLaunchKernelGGL(k_apply_merge, dim3(blocks < 4096 ? blocks : 4096), dim3(kApplyWaves * 64), 0, s, g, b,
+                     scratch_cap, scratch, actl);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_commit(const DevGraph &g, const BatchDev &b, const uint32_t *scratch,
+                               unsigned long long pool_cap, ApplyCtl *actl, const DirtyArgs &d,
+                               hipStream_t s, int only_failed) {
+  if (b.ntouched == 0) return hipSuccess;
+  const uint32_t blocks = (b.ntouched + kApplyWaves - 1) / kApplyWaves;
+  hipLaunchKernelGGL(k_apply_commit, dim3(blocks < 4096 ? blocks : 4096), dim3(kApplyWaves * 64), 0, s, g,
+                     b, scratch, pool_cap, actl, d, only_failed);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_commit(const DevGraph &g, const BatchDev &b, const uint32_t *scratch,
+                               unsigned long long pool_cap, ApplyCtl *actl, const DirtyArgs &d,
+                               hipStream_t s) {
+  return launch_apply_commit(g, b, scratch, pool_cap, actl, d, s, 0);
+}
+
+hipError_t launch_apply_commit_failed(const DevGraph &g, const BatchDev &b, const uint32_t *scratch,
+                                      unsigned long long pool_cap, ApplyCtl *actl, const DirtyArgs &d,
+                                      hipStream_t s) {
+  return launch_apply_commit(g, b, scratch, pool_cap, actl, d, s, 1);
+}
+
+hipError_t launch_relabel(const DevGraph &g, const BatchDev &b, const FilterTables &ft,
+                          const DirtyArgs &d, hipStream_t s) {
+  if (b.nrelabel == 0) return hipSuccess;
+  const uint32_t blocks = (b.nrelabel + 3) / 4;
+  hipLaunchKernelGGL(k_relabel, dim3(blocks < 4096 ? blocks : 4096), dim3(256), 0, s, g, b, ft, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_full(const DevGraph &g, const FilterTables &ft, hipStream_t s) {
+  const uint32_t groups = (g.n + 63) / 64;
+  const uint32_t blocks = (groups + 3) / 4;
+  hipLaunchKernelGGL(k_filter_full, dim3(blocks < 8192 ? blocks : 8192), dim3(256), 0, s, g, ft);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_dirty(const DevGraph &g, const FilterTables &ft, const DirtyArgs &d,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_filter_light, dim3(1024), dim3(256), 0, s, g, ft, d);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_filter_heavy, dim3(1024), dim3(256), 0, s, g, ft, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_seed(const SeedArgs &a, hipStream_t s) {
+  if (a.nrec == 0 || a.nplans == 0) return hipSuccess;
+  const uint32_t blocks = (a.nrec + 255) / 256;
+  hipLaunchKernelGGL(k_seed, dim3(blocks < 4096 ? blocks : 4096), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dfs(const DfsArgs &a, uint32_t blocks, uint32_t waves_per_block, hipStream_t s) {
+  if (a.tuples) {
+    hipLaunchKernelGGL((k_dfs<4, true>), dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+  }
+  switch (waves_per_block) {
+    case 1: hipLaunchKernelGGL((k_dfs<1, false>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_dfs<2, false>), dim3(blocks), dim3(128), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_dfs<8, false>), dim3(blocks), dim3(512), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_dfs<4, false>), dim3(blocks), dim3(256), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+int dfs_max_blocks_per_cu(uint32_t waves_per_block) {
+  switch (waves_per_block) {
+    case 1: return max_blocks<1>();
+    case 2: return max_blocks<2>();
+    case 8: return max_blocks<8>();
+    default: return max_blocks<4>();
+  }
+}
+
+hipError_t launch_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tuple_count,
+                            unsigned long long *counts, uint32_t ncounts, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(256), 0, s, ctl, actl, tuple_count, counts, ncounts);
+  return hipGetLastError();
+}
+
+hipError_t launch_scratch_reset(ApplyCtl *actl, hipStream_t s) {
+  hipLaunchKernelGGL(k_scratch_reset, dim3(1), dim3(64), 0, s, actl);
+  return hipGetLastError();
+}
+
+hipError_t launch_pool_move(const uint32_t *old_pool, uint32_t *new_pool, const uint32_t *old_start,
+                            const uint32_t *new_start, const uint32_t *deg, uint32_t n, hipStream_t s) {
+  const uint32_t blocks = (n + 3) / 4;
+  hipLaunchKernelGGL(k_pool_move, dim3(blocks < 8192 ? blocks : 8192), dim3(256), 0, s, old_pool, new_pool,
+                     old_start, new_start, deg, n);
+  return hipGetLastError();
+}
+
+}  // namespace dsm

@@ -1,0 +1,291 @@
+"""Seeded synthetic workloads for the continuous subgraph matching engine.
+
+A workload is an initial data graph (vertex labels + undirected edges), a
+stream of update batches in the DSM change format (``a|d src dst label``) and a
+set of standing queries. Everything is derived from the parameters and seeds
+below by libdsm_gen.so (csrc/gen.cpp), so the same call yields the same
+workload on any machine.
+
+Shapes follow the workload definitions of the reference's benchmark plan:
+  small   1,000 V / 8,000 uniform edges, 4 uniform labels, triangle+pendant
+          query, 100 single-edge insertions of held-out edges then 100
+          single-edge deletions.
+  rmat1m  R-MAT (a,b,c,d) = (.57,.19,.19,.05), 1,000,000 V, 10,000,000
+          undirected edges, 16 Zipf(1.0) labels; 10% of edges held out as
+          inserts plus 10% of the initial edges as deletes, interleaved, in
+          batches of 4,096; one 6-vertex query (5-cycle plus chord).
+  rmat1m_q64  the same graph and stream with 64 standing queries of 4-12
+          vertices (trees, cycles, 4-cliques, chorded cycles).
+
+Label field of an insert record: the ORIGINAL label of its source vertex. Under
+the reference update rule (cpuGraph.cpp:564) an insert relabels both endpoints
+with it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_GEN = None
+
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+
+def _lib():
+    global _GEN
+    if _GEN is not None:
+        return _GEN
+    path = os.path.join(_HERE, "libdsm_gen.so")
+    if not os.path.exists(path):
+        raise RuntimeError("libdsm_gen.so is not built; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    L.gen_edges_uniform.restype = C.c_int64
+    L.gen_edges_uniform.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, u32p, u32p]
+    L.gen_edges_rmat.restype = C.c_int64
+    L.gen_edges_rmat.argtypes = [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                 C.c_uint64, u32p, u32p]
+    L.gen_labels.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_uint64, u32p]
+    L.gen_permutation.argtypes = [C.c_uint64, C.c_uint64, u64p]
+    L.gen_csr.argtypes = [C.c_uint32, C.c_uint64, u32p, u32p, u64p, u32p]
+    L.gen_find_pattern.restype = C.c_int
+    L.gen_find_pattern.argtypes = [C.c_uint32, u64p, u32p, C.c_uint32, C.c_uint32, u32p, u32p,
+                                   C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, u32p]
+    L.gen_write_dsm.restype = C.c_int
+    L.gen_write_dsm.argtypes = [C.c_char_p, C.c_uint32, u32p, C.c_uint64, u32p, u32p, C.c_uint32,
+                                u32p, u8p, u32p, u32p, u32p]
+    _GEN = L
+    return L
+
+
+class Query:
+    def __init__(self, n, labels, edges, shape=""):
+        self.n = int(n)
+        self.labels = np.asarray(labels, np.uint32)
+        self.edges = [(int(a), int(b)) for a, b in edges]
+        self.shape = shape
+
+    @property
+    def eu(self):
+        return np.array([a for a, _ in self.edges], np.uint32)
+
+    @property
+    def ev(self):
+        return np.array([b for _, b in self.edges], np.uint32)
+
+
+class Workload:
+    """Initial graph + batches + queries. Batches are flat arrays with
+    ``bstart`` offsets: batch b is records bstart[b]:bstart[b+1]."""
+
+    def __init__(self, name, n, labels, eu, ev, bstart, add, src, dst, clab, queries):
+        self.name = name
+        self.n = int(n)
+        self.labels = labels
+        self.eu, self.ev = eu, ev
+        self.bstart = bstart
+        self.add, self.src, self.dst, self.clab = add, src, dst, clab
+        self.queries = queries
+
+    @property
+    def num_batches(self):
+        return len(self.bstart) - 1
+
+    def batch(self, b):
+        s, e = int(self.bstart[b]), int(self.bstart[b + 1])
+        return self.add[s:e], self.src[s:e], self.dst[s:e], self.clab[s:e]
+
+    def batches(self, lo=0, hi=None):
+        hi = self.num_batches if hi is None else min(hi, self.num_batches)
+        return [self.batch(b) for b in range(lo, hi)]
+
+    def csr(self):
+        L = _lib()
+        off = np.zeros(self.n + 1, np.uint64)
+        nbr = np.zeros(2 * len(self.eu), np.uint32)
+        L.gen_csr(self.n, len(self.eu), self.eu, self.ev, off, nbr)
+        return off, nbr
+
+    def write(self, data_path, query_paths):
+        L = _lib()
+        rc = L.gen_write_dsm(data_path.encode(), self.n, self.labels, len(self.eu), self.eu, self.ev,
+                             self.num_batches, self.bstart.astype(np.uint32), self.add, self.src,
+                             self.dst, self.clab)
+        if rc:
+            raise IOError(data_path)
+        for q, p in zip(self.queries, query_paths):
+            write_query(p, q)
+
+
+def write_query(path, q):
+    L = _lib()
+    z32 = np.zeros(1, np.uint32)
+    rc = L.gen_write_dsm(path.encode(), q.n, q.labels, len(q.edges), q.eu, q.ev, 0xFFFFFFFF,
+                         z32, np.zeros(1, np.uint8), z32, z32, z32)
+    if rc:
+        raise IOError(path)
+
+
+def permutation(n, seed):
+    out = np.zeros(n, np.uint64)
+    _lib().gen_permutation(n, seed, out)
+    return out
+
+
+# ----------------------------------------------------------------- shapes
+
+def shape_edges(kind, k, seed=0):
+    """Edge list of a query shape on k vertices."""
+    if kind == "tri_pendant":
+        return [(0, 1), (1, 2), (0, 2), (1, 3)]
+    if kind == "cycle5_chord":
+        # 6 vertices: a 5-cycle with the chord 0-2, plus vertex 5 pendant on 3
+        return [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (0, 2), (3, 5)]
+    if kind == "cycle5_2chords":
+        return [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (0, 2), (0, 3)]
+    if kind == "clique4":
+        return [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    if kind == "cycle":
+        return [(i, (i + 1) % k) for i in range(k)]
+    if kind == "chord_cycle":
+        return [(i, (i + 1) % k) for i in range(k)] + [(0, k // 2)]
+    if kind == "tree":
+        # random recursive tree: vertex i attaches to a seeded earlier vertex
+        p = permutation(1 << 16, seed)
+        return [(int(p[i] % i), i) for i in range(1, k)]
+    raise ValueError(kind)
+
+
+def extract_query(n, off, nbr, labels, kind, k, seed, max_start_deg=1 << 30, max_walk_deg=1 << 30,
+                  budget=200000, attempts=2000):
+    """Query of the given shape whose vertices are a randomized-walk embedding
+    in the data graph; labels copied from the data vertices it landed on."""
+    edges = shape_edges(kind, k, seed)
+    pu = np.array([a for a, _ in edges], np.uint32)
+    pv = np.array([b for _, b in edges], np.uint32)
+    out = np.zeros(k, np.uint32)
+    rc = _lib().gen_find_pattern(n, off, nbr, k, len(edges), pu, pv, seed, max_start_deg,
+                                 max_walk_deg, budget, attempts, out)
+    if rc != 0:
+        return None
+    return Query(k, labels[out], edges, shape="%s%d" % (kind, k))
+
+
+# -------------------------------------------------------------- workloads
+
+def _stream(m, n_ins, n_del, batch, seed, interleave):
+    """Indices of held-out inserts (into the full edge list) and deletes (into
+    the initial edges), cut into batches. Returns (keep_mask, order list of
+    (is_add, edge index))."""
+    perm = permutation(m, seed)
+    ins = perm[:n_ins]
+    keep = np.ones(m, bool)
+    keep[ins] = False
+    init_idx = np.nonzero(keep)[0]
+    dperm = permutation(len(init_idx), seed + 1)
+    dels = init_idx[dperm[:n_del]]
+    recs = np.concatenate([np.stack([np.ones(n_ins, np.uint64), ins]),
+                           np.stack([np.zeros(n_del, np.uint64), dels])], axis=1)
+    if interleave:
+        mix = permutation(recs.shape[1], seed + 2)
+        recs = recs[:, mix]
+    return keep, recs
+
+
+def _assemble(name, n, labels, eu, ev, keep, recs, batch, queries, seed):
+    add = recs[0].astype(np.uint8)
+    idx = recs[1].astype(np.int64)
+    # orientation of each record: seeded coin so src/dst are not always u<v
+    flip = (permutation(len(idx), seed + 3) & 1).astype(bool)
+    src = np.where(flip, ev[idx], eu[idx]).astype(np.uint32)
+    dst = np.where(flip, eu[idx], ev[idx]).astype(np.uint32)
+    clab = labels[src].astype(np.uint32)
+    nb = (len(idx) + batch - 1) // batch
+    bstart = np.minimum(np.arange(nb + 1, dtype=np.int64) * batch, len(idx)).astype(np.uint32)
+    return Workload(name, n, labels, eu[keep].copy(), ev[keep].copy(), bstart, add, src, dst, clab,
+                    queries)
+
+
+def small(seed=1):
+    """1,000 V / 8,000 uniform edges, 4 uniform labels, triangle + pendant."""
+    L = _lib()
+    n, m = 1000, 8000
+    eu = np.zeros(m, np.uint32)
+    ev = np.zeros(m, np.uint32)
+    assert L.gen_edges_uniform(n, m, seed, eu, ev) == m
+    labels = np.zeros(n, np.uint32)
+    L.gen_labels(n, 4, 0.0, seed + 10, labels)
+    keep, recs = _stream(m, 100, 100, 1, seed + 20, interleave=False)
+    off = np.zeros(n + 1, np.uint64)
+    nbr = np.zeros(2 * int(keep.sum()), np.uint32)
+    L.gen_csr(n, int(keep.sum()), eu[keep].copy(), ev[keep].copy(), off, nbr)
+    q = extract_query(n, off, nbr, labels, "tri_pendant", 4, seed + 30)
+    assert q is not None
+    return _assemble("small", n, labels, eu, ev, keep, recs, 1, [q], seed + 40)
+
+
+_RMAT_CACHE = {}
+
+
+def _rmat_graph(n, m, nl, s, seed):
+    key = (n, m, nl, s, seed)
+    if key not in _RMAT_CACHE:
+        L = _lib()
+        eu = np.zeros(m, np.uint32)
+        ev = np.zeros(m, np.uint32)
+        rc = L.gen_edges_rmat(n, m, 0.57, 0.19, 0.19, seed, eu, ev)
+        if rc != m:
+            raise RuntimeError("R-MAT generation failed: %d" % rc)
+        labels = np.zeros(n, np.uint32)
+        L.gen_labels(n, nl, s, seed + 10, labels)
+        _RMAT_CACHE.clear()
+        _RMAT_CACHE[key] = (eu, ev, labels)
+    return _RMAT_CACHE[key]
+
+
+def rmat(n=1000000, m=10000000, nl=16, zipf=1.0, batch=4096, seed=2, queries=("cycle5_chord",),
+         frac=0.10, name="rmat1m", query_specs=None):
+    """R-MAT graph with a held-out insert stream and random deletes."""
+    eu, ev, labels = _rmat_graph(n, m, nl, zipf, seed)
+    n_ins = int(m * frac)
+    keep, recs = _stream(m, n_ins, n_ins, batch, seed + 20, interleave=True)
+    L = _lib()
+    ie, iv = eu[keep].copy(), ev[keep].copy()
+    off = np.zeros(n + 1, np.uint64)
+    nbr = np.zeros(2 * len(ie), np.uint32)
+    L.gen_csr(n, len(ie), ie, iv, off, nbr)
+    specs = query_specs if query_specs is not None else [(k, 6 if k == "cycle5_chord" else 4) for k in queries]
+    qs = []
+    for i, (kind, k) in enumerate(specs):
+        q = None
+        for t in range(64):
+            q = extract_query(n, off, nbr, labels, kind, k, seed * 1000 + i * 64 + t + 30,
+                              max_start_deg=256, max_walk_deg=1024, budget=20000, attempts=200)
+            if q is not None:
+                break
+        if q is None:
+            raise RuntimeError("no embedding found for %s%d" % (kind, k))
+        qs.append(q)
+    return _assemble(name, n, labels, eu, ev, keep, recs, batch, qs, seed + 40)
+
+
+def q64_specs(seed=7):
+    """64 query shapes of 4-12 vertices: trees, cycles, 4-cliques, chorded cycles."""
+    p = permutation(4096, seed)
+    kinds = ["tree", "cycle", "clique4", "chord_cycle"]
+    specs = []
+    for i in range(64):
+        kind = kinds[i % 4]
+        if kind == "clique4":
+            k = 4
+        else:
+            k = 4 + int(p[i] % 9)  # 4..12
+        specs.append((kind, k))
+    return specs
+
+
+def rmat_q64(**kw):
+    kw.setdefault("name", "rmat1m_q64")
+    return rmat(query_specs=q64_specs(), **kw)

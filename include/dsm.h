@@ -1,0 +1,186 @@
+/*
+ * dsm.h - C ABI of the MI355X continuous subgraph matching engine.
+ *
+ * Drop-in for the GPU matching path of the reference (qxk152/DSM, CDSM): same
+ * input formats, same update rule, same match semantics (labelled, injective,
+ * non-induced embeddings, counted per inserted edge), on one GPU.
+ *
+ * Conventions: the caller owns every host buffer it passes; the engine owns all
+ * device memory. Every entry point returns an int error code (DSM_OK = 0,
+ * negative on failure) and never exits the process; dsm_last_error() gives the
+ * message. One engine is used by one host thread at a time.
+ */
+#ifndef DSM_H
+#define DSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ error codes */
+#define DSM_OK 0
+#define DSM_EINVAL -1    /* bad argument (out-of-range vertex, bad query ...) */
+#define DSM_ENOMEM -2    /* device or host allocation failed */
+#define DSM_EDEVICE -3   /* HIP runtime error */
+#define DSM_EFORMAT -4   /* malformed DSM text file */
+#define DSM_EIO -5       /* file could not be opened */
+#define DSM_ELIMIT -6    /* a hard limit was exceeded (query size, labels ...) */
+#define DSM_ETRUNC -7    /* tuple buffer too small (counts are still exact) */
+#define DSM_EADMIT -8    /* served request rejected by admission control */
+
+/* Hard limits (the reference's are MAX_VQ = 16 query vertices, defs.h:18). */
+#define DSM_MAX_QUERY_VERTICES 16
+#define DSM_MAX_QUERIES 1024
+#define DSM_MAX_LABEL 254 /* query labels must be <= this */
+
+typedef struct dsm_engine dsm_engine;
+
+/* One update record: the reference's `change` (cpuGraph.h:13-18), read from a
+ * line `a|d src dst label` (io.cu:232-242). */
+typedef struct {
+  uint32_t add;   /* 1 for 'a' (insert), 0 for 'd' (delete) */
+  uint32_t src;
+  uint32_t dst;
+  uint32_t label; /* labelChange: an insert relabels BOTH endpoints with it */
+} dsm_change;
+
+/* Initial data graph: undirected edge list (each edge once, as the e-lines of
+ * the data file) plus vertex labels. Duplicate edges collapse. */
+typedef struct {
+  uint32_t num_vertices;
+  const uint32_t *labels;
+  uint64_t num_edges;
+  const uint32_t *edge_src;
+  const uint32_t *edge_dst;
+} dsm_graph_desc;
+
+/* A connected query graph of at most DSM_MAX_QUERY_VERTICES vertices. */
+typedef struct {
+  uint32_t num_vertices;
+  const uint32_t *labels;
+  uint32_t num_edges;
+  const uint32_t *edge_src;
+  const uint32_t *edge_dst;
+} dsm_query_desc;
+
+typedef struct {
+  int device;              /* HIP device ordinal (the reference's --gpu, main.cu:43) */
+  int relabel;             /* 1: reference update rule (an insert relabels both
+                              endpoints, cpuGraph.cpp:564); 0: label field ignored */
+  int count_negatives;     /* 1: delete records are matched in the pre-batch graph */
+  uint64_t tuple_capacity; /* tuples kept per batch (0: counts only) */
+  int tuples_in_host_memory; /* 1: tuple buffer in pinned host memory (spill) */
+  uint32_t waves_per_block;  /* DFS launch shape; 0 = default */
+  uint32_t blocks_per_cu;    /* 0 = as many as occupancy allows */
+  uint32_t donate_min;       /* smallest source range split off to an idle wave (0 = default) */
+  uint32_t queue_capacity;   /* donated-task queue slots (0 = default) */
+} dsm_options;
+
+/* Per-batch statistics of the last dsm_apply_batch + dsm_match_batch. */
+typedef struct {
+  uint64_t seeds_pos, seeds_neg;   /* DFS roots seeded from insert / delete records */
+  uint64_t tasks_donated;          /* sub-ranges handed to idle waves */
+  uint64_t tuples;                 /* tuples found (may exceed the capacity) */
+  uint32_t touched_vertices;       /* vertices whose adjacency changed */
+  uint32_t relabeled_vertices;     /* vertices whose label changed */
+  uint32_t dirty_groups;           /* 64-vertex groups re-filtered */
+  uint32_t relocated_vertices;     /* adjacency lists moved to a larger slot */
+  float ms_apply, ms_filter, ms_seed, ms_dfs_pos, ms_dfs_neg; /* device time */
+  uint64_t pool_entries_used, pool_entries_capacity;
+} dsm_stats;
+
+/* ------------------------------------------------------------ engine life */
+
+void dsm_options_default(dsm_options *opt);
+
+/* Builds the device-resident graph, candidate bitmaps and per-query matching
+ * plans. Replaces the reference's set-up in main.cu:59-145: readGraphToCPU-
+ * Dynamic (io.cu:122), copyMeta (io.cu:332), decycle (decycle.cpp:6),
+ * allocateMemGPU/copyGraphToGPU (io.cu:254-302) and the first clusterFilter
+ * (filter.cu:1525). */
+int dsm_create(const dsm_graph_desc *graph, const dsm_query_desc *queries, uint32_t num_queries,
+               const dsm_options *opt, dsm_engine **out);
+
+void dsm_destroy(dsm_engine *e);
+
+/* Last error message of the engine (or of the last failed dsm_create when e is
+ * NULL). Never NULL. */
+const char *dsm_last_error(const dsm_engine *e);
+
+/* --------------------------------------------------------- per-batch path */
+
+/* Applies one update batch. With count_negatives, delete records are first
+ * matched against the pre-batch graph. Then the batch is applied in place on
+ * the device under the reference's rule (updateHostGraph, cpuGraph.cpp:521-607:
+ * symmetrise, stable (src,dst) order, set insert / erase, relabel on insert)
+ * and the candidate bitmaps are refreshed for the vertices it affects
+ * (replaces copyGraphToGPU io.cu:279, clusterFilter filter.cu:1525 and compact
+ * filter.cu:1692). Records with an endpoint >= num_vertices fail the whole
+ * batch with DSM_EINVAL before anything changes. */
+int dsm_apply_batch(dsm_engine *e, const dsm_change *changes, uint32_t num_changes);
+
+/* Matches the insert records of the last applied batch against the post-batch
+ * graph and returns, per query, the positive count (sum over insert records of
+ * the embeddings that map a query edge onto the record's edge) and the
+ * negative count (the same over delete records, in the pre-batch graph).
+ * Replaces the seeding loop of main.cu:259-329 and parallelMatch
+ * (gpu_match.cu:1121), whose result it prints as `res: N` (:1338, :1429). */
+int dsm_match_batch(dsm_engine *e, uint64_t *pos_counts, uint64_t *neg_counts);
+
+/* dsm_apply_batch followed by dsm_match_batch. */
+int dsm_process_batch(dsm_engine *e, const dsm_change *changes, uint32_t num_changes,
+                      uint64_t *pos_counts, uint64_t *neg_counts);
+
+/* Tuples found by the last dsm_match_batch (needs tuple_capacity > 0). Each
+ * tuple is (query, sign, f(u_0) .. f(u_{n-1})) in DSM_MAX_QUERY_VERTICES + 2
+ * uint32 words (sign 1 = positive, 0 = negative; unused words are
+ * 0xFFFFFFFF), in no particular order. Copies at most cap tuples; *num_out
+ * receives the number found. Returns DSM_ETRUNC if more were found than the
+ * engine's tuple_capacity could hold. */
+int dsm_get_tuples(dsm_engine *e, uint32_t *out, uint64_t cap, uint64_t *num_out);
+
+int dsm_get_stats(const dsm_engine *e, dsm_stats *out);
+
+/* ------------------------------------------------------ state inspection */
+
+/* Copies the device graph out as a sorted CSR: offsets[num_vertices + 1],
+ * neighbours[cap], labels[num_vertices]. *num_entries receives the number of
+ * neighbour entries (2x the undirected edges); returns DSM_ETRUNC if cap is too
+ * small. The reference's copyGraphToCPU (io.cu:304) is its counterpart. */
+int dsm_get_graph(dsm_engine *e, uint64_t *offsets, uint32_t *neighbours, uint64_t cap,
+                  uint32_t *labels, uint64_t *num_entries);
+
+/* Candidate bitmap of query vertex u of query q: ceil(num_vertices/64) words,
+ * bit v set iff v passes the LDF/NLF filter (filter.cu:394-566) for u. The
+ * reference keeps the equivalent in d_compact_encodings_ (main.cu:141-142). */
+int dsm_get_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words);
+
+/* Number of vertices and queries of the engine. */
+uint32_t dsm_num_vertices(const dsm_engine *e);
+uint32_t dsm_num_queries(const dsm_engine *e);
+
+/* ------------------------------------------------------- DSM text files */
+
+typedef struct dsm_file dsm_file;
+
+/* Parses a DSM text file (io.cu:122-252): `t n m`, n `v id label deg` lines in
+ * id order, m `e a b` lines, then for data files `-` and batches `b bid k`
+ * with k `a|d src dst label` lines, bid = 0,1,... in order. A v-line degree
+ * that differs from the vertex's e-line count is DSM_EFORMAT (the reference
+ * would build a corrupt CSR, io.cu:174). */
+int dsm_file_read(const char *path, int is_query, dsm_file **out);
+void dsm_file_free(dsm_file *f);
+/* Views into the parsed file; valid until dsm_file_free. */
+int dsm_file_graph(const dsm_file *f, dsm_graph_desc *graph);
+int dsm_file_query(const dsm_file *f, dsm_query_desc *query);
+uint32_t dsm_file_num_batches(const dsm_file *f);
+int dsm_file_batch(const dsm_file *f, uint32_t b, const dsm_change **changes, uint32_t *num_changes);
+const char *dsm_file_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSM_H */

@@ -1,0 +1,117 @@
+"""CPU checks of the C ABI: the library loads, exports every entry point that
+include/dsm.h declares, its own DSM parser agrees with the oracle's restatement
+of io.cu:122-252 on the fixtures, and malformed inputs fail with error codes
+(no process exit, unlike the reference's exit(1), io.cu:135-140)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from dsm_amd import engine as E
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "dsm.h")).read()
+    return sorted(set(re.findall(r"\b(dsm_[a-z_]+)\s*\(", txt)))
+
+
+def test_header_matches_binding_list():
+    assert header_functions() == sorted(E.EXPORTS)
+
+
+def test_library_exports_every_header_symbol():
+    L = E.lib()
+    for name in header_functions():
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", E.library_path()], stdout=subprocess.PIPE,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (dsm_\w+)", out))
+    assert set(header_functions()) <= exported
+
+
+@pytest.mark.parametrize("name", ["example", "small"])
+def test_parser_agrees_with_oracle(golden_dir, name):
+    gname = {"example": "example_g_01.in", "small": "small_g.in"}[name]
+    qname = {"example": "example_q_01.in", "small": "small_q.in"}[name]
+    dp = os.path.join(golden_dir, gname)
+    qp = os.path.join(golden_dir, qname)
+    f = E.DsmFile(dp, False)
+    o = O.DsmFile(dp, False)
+    n, lab, eu, ev = f.graph()
+    assert n == o.n and lab.tolist() == o.labels.tolist()
+    assert eu.tolist() == o.eu.tolist() and ev.tolist() == o.ev.tolist()
+    assert f.num_batches == o.nb
+    for b in range(o.nb):
+        a1 = f.batch(b)
+        a2 = o.batch(b)
+        for x, y in zip(a1, a2):
+            assert np.asarray(x).tolist() == np.asarray(y).tolist()
+    qf = E.DsmFile(qp, True)
+    qo = O.DsmFile(qp, True)
+    qn, ql, qu, qv = qf.graph()
+    assert (qn, ql.tolist(), qu.tolist(), qv.tolist()) == (qo.n, qo.labels.tolist(), qo.eu.tolist(), qo.ev.tolist())
+
+
+BAD = {
+    "no_header": "v 0 0 0\n",
+    "degree_mismatch": "t 2 1\nv 0 0 2\nv 1 0 1\ne 0 1\n-\n",
+    "ids_out_of_order": "t 2 1\nv 1 0 1\nv 0 0 1\ne 0 1\n-\n",
+    "edge_out_of_range": "t 2 1\nv 0 0 1\nv 1 0 1\ne 0 2\n-\n",
+    "missing_edges": "t 2 2\nv 0 0 1\nv 1 0 1\ne 0 1\n-\n",
+    "bad_change": "t 2 1\nv 0 0 1\nv 1 0 1\ne 0 1\n-\nb 0 1\nx 0 1 0\n",
+    "batch_id_gap": "t 2 1\nv 0 0 1\nv 1 0 1\ne 0 1\n-\nb 1 1\na 0 1 0\n",
+    "change_out_of_range": "t 2 1\nv 0 0 1\nv 1 0 1\ne 0 1\n-\nb 0 1\na 0 5 0\n",
+    "bad_number": "t 2 1\nv 0 0 1\nv 1 zero 1\ne 0 1\n-\n",
+}
+
+
+@pytest.mark.parametrize("case", sorted(BAD))
+def test_malformed_files_fail_with_code(tmp_path, case):
+    p = tmp_path / "bad.in"
+    p.write_text(BAD[case])
+    with pytest.raises(E.DsmError) as ei:
+        E.DsmFile(str(p), False)
+    assert ei.value.code == E.DSM_EFORMAT
+
+
+def test_missing_file_is_eio(tmp_path):
+    with pytest.raises(E.DsmError) as ei:
+        E.DsmFile(str(tmp_path / "nope.in"), False)
+    assert ei.value.code == E.DSM_EIO
+
+
+def test_whitespace_insensitive(tmp_path):
+    p = tmp_path / "ws.in"
+    p.write_text("t 3 2 v 0 1 1 v 1 2 2\n\n v 2 1 1 e 0 1 e 1 2 - b 0 2 a 0 2 7 d 1 2 0")
+    f = E.DsmFile(str(p), False)
+    n, lab, eu, ev = f.graph()
+    assert n == 3 and lab.tolist() == [1, 2, 1] and f.num_batches == 1
+    add, s, d, l = f.batch(0)
+    assert add.tolist() == [1, 0] and s.tolist() == [0, 1] and d.tolist() == [2, 2] and l.tolist() == [7, 0]
+
+
+def test_create_without_device_fails_cleanly():
+    import ctypes
+    L = E.lib()
+    cnt = ctypes.c_int(0)
+    # skip where a GPU is visible; elsewhere dsm_create must return DSM_EDEVICE, not abort
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        if hip.hipGetDeviceCount(ctypes.byref(cnt)) == 0 and cnt.value > 0:
+            pytest.skip("a GPU is visible")
+    except OSError:
+        pass
+    with pytest.raises(E.DsmError) as ei:
+        E.Engine((2, [0, 0], [0], [1]), [(2, [0, 0], [0], [1])])
+    assert ei.value.code == E.DSM_EDEVICE
+
+
+def test_cli_usage_error():
+    exe = os.path.join(ROOT, "dsm_amd", "bin", "CDSM")
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr

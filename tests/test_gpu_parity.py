@@ -1,0 +1,311 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+Bar: bit-exact. Per batch and query the uint64 positive and negative counts,
+the sorted tuple multisets, the device adjacency after the update rule and the
+LDF/NLF candidate bitmaps must equal the oracle's on the same seeded inputs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from dsm_amd import engine as E
+from dsm_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = {
+    "edge": [(0, 1)],
+    "path3": [(0, 1), (1, 2)],
+    "triangle": [(0, 1), (1, 2), (0, 2)],
+    "tri_pendant": [(0, 1), (1, 2), (0, 2), (1, 3)],
+    "square": [(0, 1), (1, 2), (2, 3), (3, 0)],
+    "star4": [(0, 1), (0, 2), (0, 3), (0, 4)],
+    "clique4": [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)],
+    "house": [(0, 1), (1, 2), (2, 3), (3, 0), (0, 4), (1, 4)],
+    "path6": [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5)],
+    "cycle6_chord": [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (5, 0), (0, 3)],
+}
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def qtuple(shape, labels):
+    e = np.array(SHAPES[shape], np.uint32)
+    return (len(labels), np.asarray(labels, np.uint32), e[:, 0].copy(), e[:, 1].copy())
+
+
+def random_graph(n, m, nl, rng):
+    edges = set()
+    while len(edges) < m:
+        a, b = rng.integers(0, n, 2)
+        if a != b:
+            edges.add((int(min(a, b)), int(max(a, b))))
+    e = np.array(sorted(edges), np.uint32)
+    return rng.integers(0, nl, n).astype(np.uint32), e[:, 0].copy(), e[:, 1].copy()
+
+
+def conflict_free_batch(n, k, present, rng, p_add=0.5):
+    """k records, no undirected pair touched twice: inserts of absent pairs,
+    deletes of present ones; updates `present`."""
+    add, src, dst = [], [], []
+    used = set()
+    pl = sorted(present)
+    while len(src) < k:
+        if rng.random() < p_add or not pl:
+            a, b = map(int, rng.integers(0, n, 2))
+            key = (min(a, b), max(a, b))
+            if a == b or key in present or key in used:
+                continue
+            used.add(key)
+            present.add(key)
+            add.append(1)
+        else:
+            key = pl[int(rng.integers(0, len(pl)))]
+            if key in used or key not in present:
+                continue
+            used.add(key)
+            present.discard(key)
+            a, b = key if rng.random() < 0.5 else key[::-1]
+            add.append(0)
+        src.append(a)
+        dst.append(b)
+    return (np.array(add, np.uint8), np.array(src, np.uint32), np.array(dst, np.uint32),
+            rng.integers(0, 4, k).astype(np.uint32))
+
+
+def oracle_queries(queries):
+    return [O.Query(n, l, u, v) for n, l, u, v in queries]
+
+
+def compare_stream(n, labels, eu, ev, queries, batches, relabel=True, tuples=False, check_graph=False,
+                   **engine_kw):
+    g = O.Graph(n, labels, eu, ev)
+    oq = oracle_queries(queries)
+    eng = E.Engine((n, labels, eu, ev), queries, relabel=relabel,
+                   tuple_capacity=(1 << 20) if tuples else 0, **engine_kw)
+    used = set(int(x) for q in queries for x in np.asarray(q[1]))
+    try:
+        for bi, (add, src, dst, lab) in enumerate(batches):
+            want = O.run_stream(g, oq, [(add, src, dst, lab)], relabel=relabel, tuples=tuples)[0]
+            pos, neg = eng.process_batch(add, src, dst, lab)
+            assert pos.tolist() == want["pos"], "batch %d positive counts" % bi
+            assert neg.tolist() == want["neg"], "batch %d negative counts" % bi
+            if tuples:
+                got = eng.tuples()
+                for qi in range(len(queries)):
+                    for sign, key in ((1, "pos_tuples"), (0, "neg_tuples")):
+                        a = sorted(map(tuple, got[(qi, sign)].tolist()))
+                        b = sorted(map(tuple, want[key][qi].tolist()))
+                        assert a == b, "batch %d query %d sign %d tuples" % (bi, qi, sign)
+            if check_graph:
+                off, nb, dlab = eng.graph()
+                ooff, onb, olab = g.export()
+                assert off.tolist() == ooff.tolist(), "batch %d offsets" % bi
+                assert np.array_equal(nb, onb), "batch %d neighbours" % bi
+                clamp = np.array([l if (l < 255 and l in used) else 255 for l in olab.tolist()], np.uint32)
+                assert np.array_equal(dlab, clamp), "batch %d labels" % bi
+                for qi, q in enumerate(oq):
+                    cand = g.candidates(q)
+                    for u in range(q.n):
+                        assert np.array_equal(eng.candidates(qi, u), cand[u].astype(bool)), \
+                            "batch %d query %d vertex %d candidates" % (bi, qi, u)
+        return eng.stats()
+    finally:
+        eng.close()
+
+
+# ----------------------------------------------------------------- fixtures
+
+def test_example_fixture(golden_dir):
+    say("case: reference example fixture")
+    exp = json.load(open(os.path.join(golden_dir, "example_expected.json")))
+    d = E.DsmFile(os.path.join(golden_dir, "example_g_01.in"), False)
+    q = E.DsmFile(os.path.join(golden_dir, "example_q_01.in"), True)
+    eng = E.Engine(d.graph(), [q.graph()], tuple_capacity=1024)
+    try:
+        pos, neg = eng.process_batch(*d.batch(0))
+        assert pos.tolist() == [exp["batch0_pos"]] and neg.tolist() == [exp["batch0_neg"]]
+        t = eng.tuples()
+        assert sorted(t[(0, 1)].tolist()) == exp["batch0_tuples"]
+        for u, vs in exp["candidates_after_batch0"].items():
+            assert np.nonzero(eng.candidates(0, int(u)))[0].tolist() == vs
+    finally:
+        eng.close()
+
+
+def test_small_workload_golden(golden_dir):
+    """1,000 V / 8,000 E, triangle+pendant, 100 inserts then 100 deletes."""
+    say("case: small workload golden fixture (200 single-record batches)")
+    exp = json.load(open(os.path.join(golden_dir, "small_expected.json")))
+    d = E.DsmFile(os.path.join(golden_dir, "small_g.in"), False)
+    q = E.DsmFile(os.path.join(golden_dir, "small_q.in"), True)
+    eng = E.Engine(d.graph(), [q.graph()], tuple_capacity=4096)
+    try:
+        for b, want in enumerate(exp["batches"]):
+            pos, neg = eng.process_batch(*d.batch(b))
+            assert (int(pos[0]), int(neg[0])) == (want["pos"], want["neg"]), "batch %d" % b
+            t = eng.tuples()
+            assert sorted(t[(0, 1)].tolist()) == want["pos_tuples"], "batch %d pos tuples" % b
+            assert sorted(t[(0, 0)].tolist()) == want["neg_tuples"], "batch %d neg tuples" % b
+        off, nb, lab = eng.graph()
+        fin = exp["final_graph"]
+        assert len(nb) == fin["entries"]
+        assert int((nb.astype(np.uint64) * 2654435761 % (1 << 32)).sum() % (1 << 61)) == fin["checksum"]
+    finally:
+        eng.close()
+
+
+# ------------------------------------------------------------ random streams
+
+@pytest.mark.parametrize("relabel", [True, False])
+def test_random_multi_query_stream(relabel):
+    say("case: random multi-query stream relabel=%s" % relabel)
+    rng = np.random.default_rng(101 if relabel else 202)
+    n, m, nl = 300, 2400, 3
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple(s, rng.integers(0, nl, len(set(x for e in SHAPES[s] for x in e))))
+               for s in sorted(SHAPES)]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batches = [conflict_free_batch(n, 40, present, rng) for _ in range(8)]
+    compare_stream(n, labels, eu, ev, queries, batches, relabel=relabel, tuples=True, check_graph=True)
+
+
+def test_dense_graph_work_donation():
+    """Dense labelled graph where single seeds have large subtrees: exercises
+    the donation queue (small donate_min)."""
+    say("case: dense graph with work donation")
+    rng = np.random.default_rng(303)
+    n, m, nl = 400, 16000, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple("clique4", [0, 0, 1, 1]), qtuple("path6", [0, 1, 0, 1, 0, 1]),
+               qtuple("house", [0, 0, 0, 1, 1])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batches = [conflict_free_batch(n, 64, present, rng) for _ in range(3)]
+    st = compare_stream(n, labels, eu, ev, queries, batches, relabel=False, donate_min=64)
+    say("  donated tasks: %d" % st["tasks_donated"])
+
+
+def test_hub_vertex_many_changes_and_pool_growth():
+    """One vertex gets > 1024 inserts in one batch (several apply rounds) and
+    the pool must grow (relocation beyond the initial reserve)."""
+    say("case: hub vertex with 3000 inserts in one batch")
+    rng = np.random.default_rng(404)
+    n = 5000
+    labels = rng.integers(0, 2, n).astype(np.uint32)
+    eu = np.arange(0, n - 1, dtype=np.uint32)
+    ev = eu + 1
+    k = 3000
+    dst = rng.choice(np.arange(2, n), k, replace=False).astype(np.uint32)
+    hub = [(np.ones(k, np.uint8), np.zeros(k, np.uint32), dst, np.zeros(k, np.uint32))]
+    # then many vertices each gaining edges beyond their slack
+    more_s = rng.integers(0, n, 80000).astype(np.uint32)
+    more_d = rng.integers(0, n, 80000).astype(np.uint32)
+    keep = more_s != more_d
+    grow = [(np.ones(int(keep.sum()), np.uint8), more_s[keep], more_d[keep],
+             np.zeros(int(keep.sum()), np.uint32))]
+    queries = [qtuple("triangle", [0, 0, 1]), qtuple("star4", [0, 1, 1, 0, 1])]
+    compare_stream(n, labels, eu, ev, queries, hub + grow, relabel=True, check_graph=True)
+
+
+def test_edge_cases_batches():
+    say("case: empty / delete-only / duplicate / self-loop / absent-edge batches")
+    rng = np.random.default_rng(505)
+    n, m = 120, 700
+    labels, eu, ev = random_graph(n, m, 2, rng)
+    queries = [qtuple("triangle", [0, 1, 0]), qtuple("edge", [1, 1])]
+    z = np.zeros(0, np.uint32)
+    present = sorted(zip(eu.tolist(), ev.tolist()))
+    a, b = present[0]
+    c, d = present[1]
+    batches = [
+        (z.astype(np.uint8), z, z, z),                                             # empty
+        (np.zeros(2, np.uint8), np.array([a, d], np.uint32), np.array([b, c], np.uint32), np.zeros(2, np.uint32)),
+        (np.ones(2, np.uint8), np.array([a, a], np.uint32), np.array([b, b], np.uint32), np.zeros(2, np.uint32)),  # duplicate insert
+        (np.ones(1, np.uint8), np.array([5], np.uint32), np.array([5], np.uint32), np.ones(1, np.uint32)),       # self loop
+        (np.zeros(1, np.uint8), np.array([7], np.uint32), np.array([8], np.uint32), np.zeros(1, np.uint32)),     # maybe absent
+    ]
+    compare_stream(n, labels, eu, ev, queries, batches, relabel=True, tuples=True, check_graph=True)
+
+
+def test_invalid_batch_leaves_state_unchanged():
+    say("case: out-of-range change rejected")
+    rng = np.random.default_rng(606)
+    n, m = 100, 500
+    labels, eu, ev = random_graph(n, m, 2, rng)
+    queries = [qtuple("triangle", [0, 0, 1])]
+    g = O.Graph(n, labels, eu, ev)
+    eng = E.Engine((n, labels, eu, ev), queries)
+    try:
+        with pytest.raises(E.DsmError) as ei:
+            eng.process_batch(np.array([1, 1], np.uint8), np.array([1, 2], np.uint32),
+                              np.array([2, n], np.uint32), np.zeros(2, np.uint32))
+        assert ei.value.code == E.DSM_EINVAL
+        present = set(zip(eu.tolist(), ev.tolist()))
+        batch = conflict_free_batch(n, 30, present, rng)
+        want = O.run_stream(g, oracle_queries(queries), [batch])[0]
+        pos, neg = eng.process_batch(*batch)
+        assert pos.tolist() == want["pos"] and neg.tolist() == want["neg"]
+    finally:
+        eng.close()
+
+
+def test_rmat_scaled_stream():
+    """R-MAT (.57,.19,.19) 20,000 V / 200,000 E, 16 Zipf labels, random-walk
+    queries of mixed shapes, batches of 1,024 mixed records."""
+    say("case: scaled R-MAT stream")
+    w = W.rmat(n=20000, m=200000, batch=1024, seed=9, name="rmat20k",
+               query_specs=[("cycle5_chord", 6), ("clique4", 4), ("tree", 5), ("cycle", 5), ("chord_cycle", 6),
+                            ("tree", 7)])
+    queries = [(q.n, q.labels, q.eu, q.ev) for q in w.queries]
+    compare_stream(w.n, w.labels, w.eu, w.ev, queries, w.batches(0, 4), relabel=True, check_graph=False)
+
+
+def test_each_query_alone_equals_in_mix():
+    say("case: alone == in the mix")
+    rng = np.random.default_rng(707)
+    n, m, nl = 500, 4000, 3
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple(s, rng.integers(0, nl, len(set(x for e in SHAPES[s] for x in e))))
+               for s in ("triangle", "square", "house", "path6")]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batches = [conflict_free_batch(n, 100, present, rng) for _ in range(3)]
+    mix = E.Engine((n, labels, eu, ev), queries)
+    alone = [E.Engine((n, labels, eu, ev), [q]) for q in queries]
+    try:
+        for b in batches:
+            pm, nm = mix.process_batch(*b)
+            for i, e in enumerate(alone):
+                pa, na = e.process_batch(*b)
+                assert (int(pa[0]), int(na[0])) == (int(pm[i]), int(nm[i]))
+    finally:
+        mix.close()
+        for e in alone:
+            e.close()
+
+
+@pytest.mark.parametrize("wpb", [1, 2, 8])
+def test_launch_shapes_agree(wpb):
+    say("case: DFS launch shape waves_per_block=%d" % wpb)
+    rng = np.random.default_rng(808)
+    n, m, nl = 300, 3000, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple("house", [0, 1, 0, 1, 0]), qtuple("clique4", [0, 0, 1, 1])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batches = [conflict_free_batch(n, 60, present, rng) for _ in range(2)]
+    compare_stream(n, labels, eu, ev, queries, batches, relabel=False, waves_per_block=wpb, donate_min=64)
+
+
+def test_tuples_in_pinned_host_memory():
+    say("case: tuple spill to pinned host memory")
+    rng = np.random.default_rng(909)
+    n, m, nl = 200, 1500, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple("triangle", [0, 1, 1])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batches = [conflict_free_batch(n, 50, present, rng) for _ in range(2)]
+    compare_stream(n, labels, eu, ev, queries, batches, tuples=True, tuples_in_host_memory=True)
