@@ -1,0 +1,242 @@
+"""Benchmark of the continuous subgraph matching path on one MI355X.
+
+Default workload: the 1M-vertex R-MAT 64-query stream (R-MAT (.57,.19,.19,.05),
+1,000,000 vertices, 10,000,000 undirected edges, 16 Zipf(1.0) labels; 10% of
+the edges held out as inserts plus 10% of the initial edges as deletes,
+interleaved, in batches of 4,096 records; 64 standing queries of 4-12
+vertices). A step is one update batch: negatives in the pre-batch graph,
+the in-place update, candidate refresh and positives in the post-batch graph,
+for all 64 queries.
+
+Legs (one JSON line on stdout, details on stderr):
+  batched  one engine, all 64 queries matched per launch (the headline value)
+  plain    the drop-in path as the reference runs it: one query per engine
+           (one CDSM process per query), each engine doing the whole batch
+           pipeline; run over the first --plain-steps timed batches
+  cpu      the CPU oracle (single core) on one batch of the same stream
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pct(xs, p):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), p)) if len(xs) else None
+
+
+def make_workload(name):
+    from dsm_amd import workloads as W
+    if name == "rmat1m_q64":
+        return W.rmat_q64()
+    if name == "rmat1m":
+        return W.rmat()
+    if name == "small":
+        return W.small()
+    raise SystemExit("unknown workload %s" % name)
+
+
+def qtuples(w):
+    return [(q.n, q.labels, q.eu, q.ev) for q in w.queries]
+
+
+def run_batched(w, warmup, steps, engine_kw):
+    from dsm_amd import engine as E
+    t0 = time.perf_counter()
+    eng = E.Engine((w.n, w.labels, w.eu, w.ev), qtuples(w), **engine_kw)
+    log("batched engine set-up %.2f s" % (time.perf_counter() - t0))
+    lat, recs = [], 0
+    agg = {"ms_dfs": 0.0, "launches": 0, "entries": 0, "loads": 0, "ms_apply": 0.0, "ms_filter": 0.0,
+           "ms_seed": 0.0, "seeds": 0, "donated": 0}
+    pos_tot = neg_tot = 0
+    per_batch = []
+    try:
+        for b in range(warmup + steps):
+            batch = w.batch(b)
+            t = time.perf_counter()
+            pos, neg = eng.process_batch(*batch)
+            dt = time.perf_counter() - t
+            st = eng.stats()
+            per_batch.append((pos.copy(), neg.copy()))
+            if b < warmup:
+                log("warmup batch %d: %.2f ms" % (b, dt * 1e3))
+                continue
+            lat.append(dt)
+            recs += len(batch[1])
+            pos_tot += int(pos.sum())
+            neg_tot += int(neg.sum())
+            agg["ms_dfs"] += st["ms_dfs_pos"] + st["ms_dfs_neg"]
+            agg["launches"] += st["dfs_launches"]
+            agg["entries"] += st["dfs_entries"]
+            agg["loads"] += st["dfs_search_loads"]
+            agg["ms_apply"] += st["ms_apply"]
+            agg["ms_filter"] += st["ms_filter"]
+            agg["ms_seed"] += st["ms_seed"]
+            agg["seeds"] += st["seeds_pos"] + st["seeds_neg"]
+            agg["donated"] += st["tasks_donated"]
+            log("batch %d: %.2f ms  pos %d neg %d  dfs %.2f ms (%d launches) apply %.2f filter %.2f seeds %d "
+                "donated %d" % (b, dt * 1e3, int(pos.sum()), int(neg.sum()), st["ms_dfs_pos"] + st["ms_dfs_neg"],
+                                st["dfs_launches"], st["ms_apply"], st["ms_filter"],
+                                st["seeds_pos"] + st["seeds_neg"], st["tasks_donated"]))
+    finally:
+        eng.close()
+    return {"lat": lat, "records": recs, "pos": pos_tot, "neg": neg_tot, "agg": agg, "per_batch": per_batch}
+
+
+def run_plain(w, warmup, steps, engine_kw):
+    """One engine per query (the reference's CDSM runs one query per process);
+    a batch is done when every query's engine has processed it."""
+    from dsm_amd import engine as E
+    nb = warmup + steps
+    lat = np.zeros(steps)
+    counts = np.zeros((nb, len(w.queries), 2), np.uint64)
+    for qi, q in enumerate(qtuples(w)):
+        eng = E.Engine((w.n, w.labels, w.eu, w.ev), [q], **engine_kw)
+        try:
+            for b in range(nb):
+                batch = w.batch(b)
+                t = time.perf_counter()
+                pos, neg = eng.process_batch(*batch)
+                dt = time.perf_counter() - t
+                counts[b, qi] = (pos[0], neg[0])
+                if b >= warmup:
+                    lat[b - warmup] += dt
+        finally:
+            eng.close()
+        if qi % 16 == 15:
+            log("plain: %d/%d queries" % (qi + 1, len(w.queries)))
+    recs = sum(len(w.batch(b)[1]) for b in range(warmup, nb))
+    return {"lat": lat.tolist(), "records": recs, "counts": counts}
+
+
+def run_cpu(w, batch_index, budget_s):
+    """CPU oracle on one batch (all queries, until the budget is spent)."""
+    from oracle import oracle as O
+    g = O.Graph(w.n, w.labels, w.eu, w.ev)
+    for b in range(batch_index):
+        add, src, dst, lab = w.batch(b)
+        g.apply(add, src, dst, lab, True)
+    oq = [O.Query(q.n, q.labels, q.eu, q.ev) for q in w.queries]
+    add, src, dst, lab = w.batch(batch_index)
+    dels = add == 0
+    t0 = time.perf_counter()
+    done = 0
+    # negatives of every query, the update, positives of every query
+    for q in oq:
+        cand = g.candidates(q)
+        g.count_records(q, src[dels], dst[dels], cand)
+        done += 1
+        if time.perf_counter() - t0 > budget_s / 2:
+            break
+    frac_neg = done / len(oq)
+    t_neg = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    g.apply(add, src, dst, lab, True)
+    t_apply = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    done = 0
+    for q in oq:
+        cand = g.candidates(q)
+        g.count_records(q, src[~dels], dst[~dels], cand)
+        done += 1
+        if time.perf_counter() - t2 > budget_s / 2:
+            break
+    frac_pos = done / len(oq)
+    t_pos = time.perf_counter() - t2
+    # extrapolate the per-query loops to all queries
+    t_batch = t_neg / frac_neg + t_apply + t_pos / frac_pos
+    return {"value": len(src) / t_batch, "seconds": t_batch,
+            "sample": "batch %d (%d records), %d/%d queries timed for negatives and %d/%d for positives, "
+                      "scaled to all queries" % (batch_index, len(src), round(frac_neg * len(oq)), len(oq),
+                                                 round(frac_pos * len(oq)), len(oq))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat1m_q64")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--waves-per-block", type=int, default=4)
+    ap.add_argument("--donate-min", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the engine is single-GPU (multi-GPU sharding is design-only); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    t = time.perf_counter()
+    w = make_workload(args.workload)
+    log("workload %s generated in %.1f s: %d vertices, %d initial edges, %d batches, %d queries"
+        % (w.name, time.perf_counter() - t, w.n, len(w.eu), w.num_batches, len(w.queries)))
+    engine_kw = {"waves_per_block": args.waves_per_block, "donate_min": args.donate_min}
+
+    bat = run_batched(w, args.warmup, args.steps, engine_kw)
+    total_s = sum(bat["lat"])
+    value = bat["records"] / total_s
+    ms_per_step = total_s / len(bat["lat"]) * 1e3
+    agg = bat["agg"]
+    # roofline of the dominant kernel (the DFS): algorithmic bytes per launch =
+    # entries * (4 B list read + 8 B bitmap word) + search loads * 4 B
+    dfs_bytes = agg["entries"] * 12 + agg["loads"] * 4
+    dfs_ms_avg = agg["ms_dfs"] / max(agg["launches"], 1)
+    bytes_per_launch = dfs_bytes / max(agg["launches"], 1)
+    achieved = bytes_per_launch / (dfs_ms_avg * 1e-3) / 1e9 if dfs_ms_avg > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": "k_dfs", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(dfs_ms_avg, 4)}
+
+    out = {"metric": "edge updates/s + p50/p99 batch latency, served/batched vs plain GPU path",
+           "value": round(value, 1), "unit": "edge updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+           "config": {"workload": "%s: R-MAT 1M V / 10M E, 16 Zipf labels, 4,096-record batches, %d standing "
+                                  "queries, batched (one persistent launch per graph state)"
+                                  % (w.name, len(w.queries)),
+                      "batch": 4096, "queries": len(w.queries), "mode": "batched",
+                      "parallelism": "single GPU"},
+           "p50_batch_ms": round(pct(bat["lat"], 50) * 1e3, 3), "p99_batch_ms": round(pct(bat["lat"], 99) * 1e3, 3),
+           "matches_per_s": round((bat["pos"] + bat["neg"]) / total_s, 1),
+           "stages_ms_per_batch": {k: round(agg[k] / len(bat["lat"]), 3) for k in ("ms_apply", "ms_filter",
+                                                                               "ms_seed", "ms_dfs")},
+           "roofline": roofline}
+
+    if not args.no_plain:
+        ps = min(args.plain_steps, args.steps)
+        pl = run_plain(w, args.warmup, ps, engine_kw)
+        # the plain path must produce the same counts
+        for b in range(args.warmup + ps):
+            bp, bn = bat["per_batch"][b]
+            assert np.array_equal(pl["counts"][b, :, 0], bp) and np.array_equal(pl["counts"][b, :, 1], bn), \
+                "plain and batched counts differ at batch %d" % b
+        plain_value = pl["records"] / sum(pl["lat"])
+        out["plain"] = {"value": round(plain_value, 1), "steps": ps,
+                        "p50_batch_ms": round(pct(pl["lat"], 50) * 1e3, 3),
+                        "p99_batch_ms": round(pct(pl["lat"], 99) * 1e3, 3)}
+        out["batched_vs_plain"] = round(value / plain_value, 3)
+
+    if not args.no_cpu:
+        cpu = run_cpu(w, args.warmup, args.cpu_budget)
+        out["cpu_baseline"] = {"value": round(cpu["value"], 2), "unit": "edge updates/s", "cores": 1,
+                               "kind": "port", "sample": cpu["sample"]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -133,6 +133,8 @@ struct dsm_engine {
   bool tuples_host = false;
   unsigned long long tuple_cap = 0;
   uint32_t dfs_blocks = 0, dfs_wpb = 4;
+  uint32_t watchdog_ms = 60000;
+  int wall_khz = 100000;
   // records of the last batch (file order)
   std::vector<uint32_t> pos_s, pos_d, neg_s, neg_d;
   DevBuf<uint32_t> rec_neg, rec_pos;  // records on the device: src[k] | dst[k]
@@ -511,6 +513,7 @@ int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nr
   da.epoch = ++e->epoch;
   da.donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
   da.sign = sign;
+  da.watchdog = (unsigned long long)e->wall_khz * e->watchdog_ms;
   da.ctl = e->ctl.p;
   da.counts = e->counts.p;
   da.tuples = e->tuples;
@@ -524,9 +527,16 @@ int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nr
 
 int check_round(dsm_engine *e, uint64_t *seeds) {
   const DfsCtl &c = *e->h_ctl.p;
-  if (c.seed_overflow) return e->fail_msg(DSM_ELIMIT, "seed table overflow (%u seeds dropped)", c.seed_overflow);
-  *seeds = c.nseeds;
-  e->stats.tasks_donated += c.donated;
+  if (c.seed_overflow.v)
+    return e->fail_msg(DSM_ELIMIT, "seed table overflow (%u seeds dropped)", c.seed_overflow.v);
+  if (c.abort.v)
+    return e->fail_msg(DSM_EDEVICE, "DFS watchdog: a wave waited more than %u ms for work (%d tasks left)",
+                       e->watchdog_ms, c.work_left.v);
+  *seeds = c.nseeds.v;
+  e->stats.tasks_donated += c.q_tail.v;
+  e->stats.dfs_entries += c.scanned.v;
+  e->stats.dfs_search_loads += c.loads.v;
+  e->stats.dfs_launches += 1;
   return DSM_OK;
 }
 
@@ -742,6 +752,9 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   HIPCHK(hipSetDevice(e->opt.device));
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   HIPCHK(hipDeviceGetAttribute(&e->num_cu, hipDeviceAttributeMultiprocessorCount, e->opt.device));
+  HIPCHK(hipDeviceGetAttribute(&e->wall_khz, hipDeviceAttributeWallClockRate, e->opt.device));
+  if (e->wall_khz <= 0) e->wall_khz = 100000;
+  if (e->opt.watchdog_ms) e->watchdog_ms = e->opt.watchdog_ms;
   for (auto &x : e->ev) HIPCHK(hipEventCreate(&x));
   if ((rc = build_queries(e, queries, num_queries))) return rc;
   if ((rc = upload_graph(e, graph))) return rc;

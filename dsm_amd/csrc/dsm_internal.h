@@ -46,17 +46,25 @@ struct alignas(16) Task {
 };
 constexpr int kTaskWords = sizeof(Task) / 4;  // 20
 
-// Device-resident control block of one seed + DFS round.
-struct alignas(64) DfsCtl {
-  uint32_t nseeds;        // seeds written by the seed kernel
-  uint32_t seed_head;     // next seed to claim
-  uint32_t q_head;        // next queue slot to claim
-  uint32_t q_tail;        // next queue slot to fill
-  int32_t work_left;      // seeds + donated tasks not yet finished
-  uint32_t idle;          // waves waiting for work
-  uint32_t donated;       // tasks donated this round
-  uint32_t seed_overflow; // seeds dropped for lack of capacity (must stay 0)
-  uint32_t pad[8];
+// Device-resident control block of one seed + DFS round. Every counter that
+// waves update or poll concurrently has a 128-byte line of its own, so polling
+// by idle waves does not serialise the atomics of busy ones.
+template <typename T>
+struct alignas(128) CtlLine {
+  T v;
+  uint32_t pad[(128 - sizeof(T)) / 4];
+};
+struct DfsCtl {
+  CtlLine<uint32_t> nseeds;        // seeds written by the seed kernel
+  CtlLine<uint32_t> seed_head;     // next seed to claim
+  CtlLine<uint32_t> q_head;        // next queue slot to claim
+  CtlLine<uint32_t> q_tail;        // next queue slot to fill (= tasks donated)
+  CtlLine<int32_t> work_left;      // seeds + donated tasks not yet finished
+  CtlLine<uint32_t> idle;          // waves waiting for work
+  CtlLine<uint32_t> seed_overflow; // seeds dropped for lack of capacity (must stay 0)
+  CtlLine<uint32_t> abort;         // set when an idle wave outlived the watchdog
+  CtlLine<unsigned long long> scanned;  // source-list entries tested
+  CtlLine<unsigned long long> loads;    // 4-byte reads of membership searches
 };
 
 // Status of the apply kernels of one batch.
@@ -120,6 +128,7 @@ struct DfsArgs {
   uint32_t epoch;
   uint32_t donate_min;
   uint32_t sign;                 // 1 positive, 0 negative
+  unsigned long long watchdog;   // wall-clock ticks an idle wave may wait
   DfsCtl *ctl;
   unsigned long long *counts;    // [nq][2]: index q*2+sign
   uint32_t *tuples;              // [tuple_cap][kMaxQ+2] or null

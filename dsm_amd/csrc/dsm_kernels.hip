@@ -22,7 +22,7 @@ namespace {
 
 constexpr int kApplyWaves = 4;
 constexpr int kLight = 64;          // dirty vertices up to this degree: one lane each
-constexpr uint32_t kDonateEvery = 8;  // chunks between idle checks
+constexpr uint32_t kDonateEvery = 16;  // chunks between idle checks
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
@@ -426,12 +426,12 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
     // d in N(s): the direction the DFS also tests (w in the list of an
     // earlier image); the two coincide whenever the graph is symmetric
     if (!contains(a.g.pool + a.g.start[s], a.g.deg[s], d)) continue;
-    const uint32_t base = atomicAdd(&a.ctl->nseeds, cnt);
+    const uint32_t base = atomicAdd(&a.ctl->nseeds.v, cnt);
     if (base + cnt > a.seed_cap) {
-      atomicAdd(&a.ctl->seed_overflow, cnt);
+      atomicAdd(&a.ctl->seed_overflow.v, cnt);
       continue;
     }
-    atomicAdd(&a.ctl->work_left, (int)cnt);
+    atomicAdd(&a.ctl->work_left.v, (int)cnt);
     uint32_t o = base;
     for (uint32_t p = p0; p < p1; p++) {
       const Plan &pl = a.plans[p];
@@ -451,14 +451,16 @@ __global__ void k_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tup
   if (threadIdx.x == 0) {
     if (tuple_count) *tuple_count = 0;
     if (ctl) {
-      ctl->nseeds = 0;
-      ctl->seed_head = 0;
-      ctl->q_head = 0;
-      ctl->q_tail = 0;
-      ctl->work_left = 0;
-      ctl->idle = 0;
-      ctl->donated = 0;
-      ctl->seed_overflow = 0;
+      ctl->nseeds.v = 0;
+      ctl->seed_head.v = 0;
+      ctl->q_head.v = 0;
+      ctl->q_tail.v = 0;
+      ctl->work_left.v = 0;
+      ctl->idle.v = 0;
+      ctl->seed_overflow.v = 0;
+      ctl->abort.v = 0;
+      ctl->scanned.v = 0;
+      ctl->loads.v = 0;
     }
     if (actl) {
       actl->scratch_used = 0;
@@ -487,6 +489,8 @@ struct WaveLds {
   uint32_t end[kMaxQ];       // one past the last source index
   uint32_t cn[kMaxQ];        // chunk size
   uint32_t ci[kMaxQ];        // next chunk entry
+  uint32_t cbase[kMaxQ];     // source index of the chunk's first lane
+  uint64_t cmask[kMaxQ];     // lanes of the chunk that passed (k-th set bit = cand[k])
   uint16_t sig[kMaxQ];
   uint16_t back[kMaxQ];
   uint16_t inj[kMaxQ];
@@ -525,7 +529,10 @@ __device__ __forceinline__ void init_level(WaveLds &S, uint32_t l, uint32_t lo, 
 // Is w a valid image for position l: candidate bit, injectivity against
 // earlier positions of the same label, adjacency to every backward neighbour
 // other than the one whose list produced w.
-__device__ __forceinline__ bool check(const WaveLds &S, const DfsArgs &a, uint32_t l, uint32_t w) {
+// `loads` counts the 4-byte list reads of the membership searches (for the
+// algorithmic-bytes accounting of the roofline).
+__device__ __forceinline__ bool check(const WaveLds &S, const DfsArgs &a, uint32_t l, uint32_t w,
+                                      uint32_t &loads) {
   if (!bit_of(a.bm, a.words, S.sig[l], w)) return false;
   uint32_t inj = S.inj[l];
   while (inj) {
@@ -537,7 +544,9 @@ __device__ __forceinline__ bool check(const WaveLds &S, const DfsArgs &a, uint32
   while (oth) {
     const uint32_t j = __builtin_ctz(oth);
     oth &= oth - 1;
-    if (!contains(a.g.pool + S.fs[j], S.fd[j], w)) return false;
+    const uint32_t d = S.fd[j];
+    loads += d ? 32u - (uint32_t)__builtin_clz(d) : 0u;
+    if (!contains(a.g.pool + S.fs[j], d, w)) return false;
   }
   return true;
 }
@@ -561,30 +570,64 @@ __device__ void emit_tuples(const WaveLds &S, const DfsArgs &a, uint32_t l, uint
 
 // Hands half of the shallowest large-enough remaining source range to the
 // queue when some wave is idle.
-__device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l) {
+//
+// Levels shallower than the current one give away work when at least two
+// candidates remain there (fetched-but-unconsumed chunk entries count: their
+// source indices are recovered from the chunk mask); the level being scanned
+// only when at least donate_min source entries remain. The receiver re-fetches
+// and re-filters the donated source range [mid, end).
+__device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l, uint32_t n) {
   uint32_t idle = 0;
-  if (lane_id() == 0) idle = ld_relaxed(&a.ctl->idle);
+  if (lane_id() == 0) idle = ld_relaxed(&a.ctl->idle.v);
   idle = bcast(idle);
   if (idle == 0) return;
-  uint32_t lv = 0xFFFFFFFFu;
+  uint32_t lv = 0xFFFFFFFFu, mid = 0, keep_cn = 0;
   for (uint32_t k = L0; k <= l; k++) {
     const uint32_t p = S.pos[k], e = S.end[k];
-    if (e > p && e - p >= a.donate_min) {
+    const uint32_t unf = e > p ? e - p : 0;
+    if (k == l) {
+      if (unf >= a.donate_min) {
+        lv = k;
+        mid = p + unf / 2;
+      }
+      break;
+    }
+    const uint32_t ci = S.ci[k], cn = S.cn[k];
+    const uint32_t rem = cn - ci;
+    // an entry two or more levels above the leaf roots a subtree worth a
+    // hand-off; one level above, only a leaf scan
+    const uint32_t need = (k + 3 <= n) ? 2u : 16u;
+    if (rem + unf < need) continue;
+    if (unf >= 2) {
       lv = k;
+      mid = p + unf / 2;
+      keep_cn = cn;
+      break;
+    }
+    {
+      // split the unconsumed chunk: keep [ci, ci+half), donate from the
+      // source index of entry ci+half to the end of the range
+      const uint32_t half = (rem + unf) / 2;
+      const uint32_t kth = ci + half;
+      if (half == 0 || kth >= cn) continue;
+      uint64_t m = S.cmask[k];
+      for (uint32_t i = 0; i < kth; i++) m &= m - 1;
+      lv = k;
+      mid = S.cbase[k] + (uint32_t)__builtin_ctzll(m);
+      keep_cn = kth;
       break;
     }
   }
   if (lv == 0xFFFFFFFFu) return;
-  const uint32_t p = S.pos[lv], e = S.end[lv];
-  const uint32_t mid = p + (e - p) / 2;
+  (void)n;
+  const uint32_t e = S.end[lv];
   uint32_t slot = 0, ok = 0;
   if (lane_id() == 0) {
-    const uint32_t t = ld_relaxed(&a.ctl->q_tail);
-    if (t < a.qcap && cas_u32(&a.ctl->q_tail, t, t + 1)) {
+    const uint32_t t = ld_relaxed(&a.ctl->q_tail.v);
+    if (t < a.qcap && cas_u32(&a.ctl->q_tail.v, t, t + 1)) {
       ok = 1;
       slot = t;
-      atomicAdd(&a.ctl->work_left, 1);
-      atomicAdd(&a.ctl->donated, 1u);
+      atomicAdd(&a.ctl->work_left.v, 1);
     }
   }
   ok = bcast(ok);
@@ -601,6 +644,10 @@ __device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   if (lane == 0) {
     st_release(&a.ready[slot], a.epoch);
+    // the donor keeps source indices below mid; after a split inside the
+    // chunk, mid <= pos, so nothing is left to fetch at this level and a later
+    // split there donates [mid', mid) only
+    if (lv != l && keep_cn != S.cn[lv]) S.cn[lv] = keep_cn;
     S.end[lv] = mid;
   }
   wave_sync();
@@ -616,51 +663,83 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
   bool idle = false;
   // a seed table overflow leaves unwritten slots: match nothing, the host
   // reports the error (the table is sized so that it cannot happen)
-  if (a.ctl->seed_overflow) return;
-  const uint32_t nseeds = a.ctl->nseeds;
+  if (a.ctl->seed_overflow.v) return;
+  const uint32_t nseeds = a.ctl->nseeds.v;
+  bool seeds_left = true;
+  uint32_t naps = 0;            // idle polls since the last task (backoff)
+  uint64_t idle_since = 0;      // wall clock when the wave went idle (watchdog)
   for (;;) {
     uint32_t got = 0, slot = 0;
     if (lane == 0) {
-      uint32_t h = ld_relaxed(&a.ctl->q_head);
-      while (h < ld_relaxed(&a.ctl->q_tail)) {
-        if (cas_u32(&a.ctl->q_head, h, h + 1)) {
+      uint32_t h = ld_relaxed(&a.ctl->q_head.v);
+      while (h < ld_relaxed(&a.ctl->q_tail.v)) {
+        if (cas_u32(&a.ctl->q_head.v, h, h + 1)) {
           got = 1;
           slot = h;
           break;
         }
-        h = ld_relaxed(&a.ctl->q_head);
+        h = ld_relaxed(&a.ctl->q_head.v);
       }
-      if (!got && ld_relaxed(&a.ctl->seed_head) < nseeds) {
-        const uint32_t i = atomicAdd(&a.ctl->seed_head, 1u);
+      if (!got && seeds_left) {
+        const uint32_t i = atomicAdd(&a.ctl->seed_head.v, 1u);
         if (i < nseeds) {
           got = 2;
           slot = i;
+        } else {
+          seeds_left = false;
         }
       }
     }
     got = bcast(got);
     slot = bcast(slot);
     if (got == 0) {
-      int32_t left = 0;
-      if (lane == 0) left = ld_relaxed(&a.ctl->work_left);
-      left = (int32_t)bcast((uint32_t)left);
-      if (left <= 0) break;
+      // idle: poll with exponential backoff; the termination counter and the
+      // abort flag only every fourth poll
       if (!idle) {
-        if (lane == 0) atomicAdd(&a.ctl->idle, 1u);
+        if (lane == 0) atomicAdd(&a.ctl->idle.v, 1u);
         idle = true;
+        naps = 0;
+        idle_since = wall_clock64();
       }
-      __builtin_amdgcn_s_sleep(16);
+      if ((naps & 3) == 0) {
+        int32_t left = 0;
+        uint32_t ab = 0;
+        if (lane == 0) {
+          left = ld_relaxed(&a.ctl->work_left.v);
+          ab = ld_relaxed(&a.ctl->abort.v);
+        }
+        left = (int32_t)bcast((uint32_t)left);
+        ab = bcast(ab);
+        if (left <= 0 || ab) break;
+        if (wall_clock64() - idle_since > a.watchdog) {
+          if (lane == 0) st_release(&a.ctl->abort.v, 1u);
+          break;
+        }
+      }
+      const uint32_t reps = naps < 6 ? (1u << naps) : 64u;
+      for (uint32_t r = 0; r < reps; r++) __builtin_amdgcn_s_sleep(8);
+      naps++;
       continue;
     }
     if (idle) {
-      if (lane == 0) atomicSub(&a.ctl->idle, 1u);
+      if (lane == 0) atomicSub(&a.ctl->idle.v, 1u);
       idle = false;
     }
     // ---- load the task
     uint32_t plan, L0, lo, hi, fw = 0;
     if (got == 1) {
-      if (lane == 0)
-        while (ld_acquire(&a.ready[slot]) != a.epoch) __builtin_amdgcn_s_sleep(1);
+      if (lane == 0) {
+        // the donor publishes right after reserving the slot; bounded anyway
+        const uint64_t t0 = wall_clock64();
+        while (ld_acquire(&a.ready[slot]) != a.epoch) {
+          __builtin_amdgcn_s_sleep(1);
+          if (wall_clock64() - t0 > a.watchdog) {
+            st_release(&a.ctl->abort.v, 1u);
+            break;
+          }
+        }
+      }
+      if (bcast(ld_relaxed(&a.ctl->abort.v))) break;
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       const uint32_t word =
           lane < (uint32_t)kTaskWords ? reinterpret_cast<const uint32_t *>(&a.queue[slot])[lane] : 0u;
@@ -698,6 +777,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
     wave_sync();
     const uint32_t n = bcast(S.n);
     uint64_t lcount = 0;
+    uint32_t scanned = 0, loads = 0;  // per lane: source entries read, search loads
     if (L0 >= n) {
       // complete embedding (two-vertex query seeded by its only edge)
       if (lane == 0) lcount = 1;
@@ -712,7 +792,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
           for (;;) {
             if (++tick >= kDonateEvery) {
               tick = 0;
-              try_donate(S, a, L0, l);
+              try_donate(S, a, L0, l, n);
             }
             const uint32_t p = bcast(S.pos[l]), e = bcast(S.end[l]);
             if (p >= e) break;
@@ -721,7 +801,8 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
             uint32_t w = 0;
             if (i < e) {
               w = a.g.pool[S.fs[S.srcj[l]] + i];
-              ok = check(S, a, l, w);
+              ok = check(S, a, l, w, loads);
+              scanned++;
             }
             lcount += ok ? 1u : 0u;
             if (TUP) emit_tuples(S, a, l, w, ok);
@@ -745,7 +826,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
         }
         if (++tick >= kDonateEvery) {
           tick = 0;
-          try_donate(S, a, L0, l);
+          try_donate(S, a, L0, l, n);
         }
         const uint32_t p = bcast(S.pos[l]), e = bcast(S.end[l]);
         if (p < e) {
@@ -754,7 +835,8 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
           uint32_t w = 0;
           if (i < e) {
             w = a.g.pool[S.fs[S.srcj[l]] + i];
-            ok = check(S, a, l, w);
+            ok = check(S, a, l, w, loads);
+            scanned++;
           }
           const uint64_t m = ballot(ok);
           wave_sync();
@@ -762,6 +844,8 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
           if (lane == 0) {
             S.cn[l] = __popcll(m);
             S.ci[l] = 0;
+            S.cbase[l] = p;
+            S.cmask[l] = m;
             S.pos[l] = p + 64;
           }
           wave_sync();
@@ -773,10 +857,17 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
     }
     // wave sum of the lane counts
     uint64_t tot = lcount;
-    for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off);
+    uint64_t sc = scanned, ld = loads;
+    for (int off = 32; off >= 1; off >>= 1) {
+      tot += __shfl_xor(tot, off);
+      sc += __shfl_xor(sc, off);
+      ld += __shfl_xor(ld, off);
+    }
     if (lane == 0) {
       if (tot) atomicAdd(&a.counts[(size_t)bcast(S.query) * 2 + a.sign], (unsigned long long)tot);
-      atomicSub(&a.ctl->work_left, 1);
+      atomicAdd(&a.ctl->scanned.v, (unsigned long long)sc);
+      atomicAdd(&a.ctl->loads.v, (unsigned long long)ld);
+      atomicSub(&a.ctl->work_left.v, 1);
     }
   }
 }

@@ -161,11 +161,14 @@ void gen_csr(uint32_t n, uint64_t m, const uint32_t *eu, const uint32_t *ev, uin
 // start vertex, then for each pattern vertex (in BFS order from pattern vertex
 // 0) a random neighbour of its mapped parent that closes every pattern edge to
 // already-mapped vertices. Start vertices with degree above max_start_deg are
-// skipped; each start gets `budget` extension steps. Writes the data vertex of
+// skipped, and the walk only visits vertices with label >= min_label (so that
+// queries can be drawn over rarer labels); each start gets `budget` extension
+// steps. Writes the data vertex of
 // each pattern vertex to out. Returns 0 on success, -1 when no start succeeds.
-int gen_find_pattern(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint32_t k, uint32_t pm,
-                     const uint32_t *pu, const uint32_t *pv, uint64_t seed, uint32_t max_start_deg,
-                     uint32_t max_walk_deg, uint64_t budget, uint32_t attempts, uint32_t *out) {
+int gen_find_pattern(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab,
+                     uint32_t min_label, uint32_t k, uint32_t pm, const uint32_t *pu, const uint32_t *pv,
+                     uint64_t seed, uint32_t max_start_deg, uint32_t max_walk_deg, uint64_t budget,
+                     uint32_t attempts, uint32_t *out) {
   if (k == 0 || k > 32) return -1;
   std::vector<uint32_t> adj(k, 0);
   for (uint32_t i = 0; i < pm; i++) { adj[pu[i]] |= 1u << pv[i]; adj[pv[i]] |= 1u << pu[i]; }
@@ -183,7 +186,7 @@ int gen_find_pattern(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint3
   for (uint32_t at = 0; at < attempts; at++) {
     uint32_t v0 = (uint32_t)r.below(n);
     uint64_t d0 = off[v0 + 1] - off[v0];
-    if (d0 == 0 || d0 > max_start_deg) continue;
+    if (d0 == 0 || d0 > max_start_deg || lab[v0] < min_label) continue;
     f[order[0]] = v0;
     uint64_t steps = 0;
     int lvl = 1;
@@ -203,7 +206,7 @@ int gen_find_pattern(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint3
       while (iter[lvl] < c.size()) {
         uint32_t w = c[iter[lvl]++];
         steps++;
-        if (off[w + 1] - off[w] > max_walk_deg) continue;
+        if (off[w + 1] - off[w] > max_walk_deg || lab[w] < min_label) continue;
         bool ok = true;
         for (int j = 0; j < lvl && ok; j++)
           if (f[order[j]] == w) ok = false;
@@ -227,6 +230,78 @@ int gen_find_pattern(uint32_t n, const uint64_t *off, const uint32_t *nbr, uint3
     }
   }
   return -1;
+}
+
+// Estimated number of embeddings of a labelled query that use one random data
+// edge (s,d) (uniform over the 2m directed adjacency entries), by Knuth's
+// estimator of backtracking-tree size: each probe draws an edge and an ordered
+// query edge (u,v), maps u->s, v->d, then extends the remaining query vertices
+// in a connected order, picking one valid extension uniformly at random and
+// multiplying the number of valid extensions seen at every level. The mean of
+// the products, times the number of ordered query edges, is an unbiased
+// estimate of the embeddings seeded per edge record. Returns -1 on bad input.
+double gen_estimate_per_edge(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab,
+                             uint32_t k, uint32_t pm, const uint32_t *pu, const uint32_t *pv,
+                             const uint32_t *qlab, uint32_t probes, uint64_t seed) {
+  if (k < 2 || k > 32 || pm == 0 || off[n] == 0) return -1.0;
+  std::vector<uint32_t> adj(k, 0);
+  std::vector<std::pair<uint32_t, uint32_t>> qe;
+  for (uint32_t i = 0; i < pm; i++) {
+    adj[pu[i]] |= 1u << pv[i];
+    adj[pv[i]] |= 1u << pu[i];
+    qe.push_back({pu[i], pv[i]});
+    qe.push_back({pv[i], pu[i]});
+  }
+  Rng r(seed);
+  double total = 0;
+  std::vector<uint32_t> f(k), cand;
+  for (uint32_t pr = 0; pr < probes; pr++) {
+    const uint64_t ent = r.below(off[n]);
+    const uint32_t s = (uint32_t)(std::upper_bound(off, off + n + 1, ent) - off - 1);
+    const uint32_t d = nbr[ent];
+    const auto &e = qe[r.below(qe.size())];
+    if (lab[s] != qlab[e.first] || lab[d] != qlab[e.second] || s == d) continue;
+    uint32_t placed = (1u << e.first) | (1u << e.second);
+    f[e.first] = s;
+    f[e.second] = d;
+    double prod = 1.0;
+    for (uint32_t step = 2; step < k && prod > 0; step++) {
+      // next: the unplaced vertex with the most placed neighbours
+      int best = -1, bb = -1;
+      for (uint32_t w = 0; w < k; w++) {
+        if (placed >> w & 1) continue;
+        int b = __builtin_popcount(adj[w] & placed);
+        if (b > bb) { bb = b; best = (int)w; }
+      }
+      const uint32_t u = (uint32_t)best;
+      // scan the shortest backward list
+      uint32_t src = 0;
+      uint64_t sdeg = ~0ull;
+      for (uint32_t j = 0; j < k; j++)
+        if ((placed >> j & 1) && (adj[u] >> j & 1)) {
+          uint64_t dg = off[f[j] + 1] - off[f[j]];
+          if (dg < sdeg) { sdeg = dg; src = j; }
+        }
+      cand.clear();
+      for (uint64_t i = off[f[src]]; i < off[f[src] + 1]; i++) {
+        const uint32_t w = nbr[i];
+        if (lab[w] != qlab[u]) continue;
+        bool ok = true;
+        for (uint32_t j = 0; j < k && ok; j++) {
+          if (!(placed >> j & 1)) continue;
+          if (f[j] == w) ok = false;
+          else if (j != src && (adj[u] >> j & 1) && !std::binary_search(nbr + off[w], nbr + off[w + 1], f[j])) ok = false;
+        }
+        if (ok) cand.push_back(w);
+      }
+      prod *= (double)cand.size();
+      if (cand.empty()) break;
+      f[u] = cand[r.below(cand.size())];
+      placed |= 1u << u;
+    }
+    total += prod;
+  }
+  return total / probes * (double)qe.size();
 }
 
 // Writes a data graph + update batches (or, with nb == UINT32_MAX, a query

@@ -63,7 +63,7 @@ class Options(C.Structure):
     _fields_ = [("device", C.c_int), ("relabel", C.c_int), ("count_negatives", C.c_int),
                 ("tuple_capacity", C.c_uint64), ("tuples_in_host_memory", C.c_int),
                 ("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("donate_min", C.c_uint32), ("queue_capacity", C.c_uint32)]
+                ("donate_min", C.c_uint32), ("queue_capacity", C.c_uint32), ("watchdog_ms", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -72,7 +72,8 @@ class Stats(C.Structure):
                 ("relabeled_vertices", C.c_uint32), ("dirty_groups", C.c_uint32),
                 ("relocated_vertices", C.c_uint32), ("ms_apply", C.c_float), ("ms_filter", C.c_float),
                 ("ms_seed", C.c_float), ("ms_dfs_pos", C.c_float), ("ms_dfs_neg", C.c_float),
-                ("pool_entries_used", C.c_uint64), ("pool_entries_capacity", C.c_uint64)]
+                ("pool_entries_used", C.c_uint64), ("pool_entries_capacity", C.c_uint64),
+                ("dfs_entries", C.c_uint64), ("dfs_search_loads", C.c_uint64), ("dfs_launches", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -200,7 +201,7 @@ class Engine:
 
     def __init__(self, graph, queries, device=0, relabel=True, count_negatives=True, tuple_capacity=0,
                  tuples_in_host_memory=False, waves_per_block=4, blocks_per_cu=0, donate_min=0,
-                 queue_capacity=0):
+                 queue_capacity=0, watchdog_ms=0):
         L = lib()
         n, lab, eu, ev = graph
         self._keep = []
@@ -223,6 +224,7 @@ class Engine:
         opt.blocks_per_cu = blocks_per_cu
         opt.donate_min = donate_min
         opt.queue_capacity = queue_capacity
+        opt.watchdog_ms = watchdog_ms
         h = C.c_void_p()
         rc = L.dsm_create(C.byref(gd), qds, len(queries), C.byref(opt), C.byref(h))
         self._keep = None

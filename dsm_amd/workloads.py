@@ -51,8 +51,11 @@ def _lib():
     L.gen_permutation.argtypes = [C.c_uint64, C.c_uint64, u64p]
     L.gen_csr.argtypes = [C.c_uint32, C.c_uint64, u32p, u32p, u64p, u32p]
     L.gen_find_pattern.restype = C.c_int
-    L.gen_find_pattern.argtypes = [C.c_uint32, u64p, u32p, C.c_uint32, C.c_uint32, u32p, u32p,
-                                   C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, u32p]
+    L.gen_find_pattern.argtypes = [C.c_uint32, u64p, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p,
+                                   u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, u32p]
+    L.gen_estimate_per_edge.restype = C.c_double
+    L.gen_estimate_per_edge.argtypes = [C.c_uint32, u64p, u32p, u32p, C.c_uint32, C.c_uint32, u32p, u32p,
+                                        u32p, C.c_uint32, C.c_uint64]
     L.gen_write_dsm.restype = C.c_int
     L.gen_write_dsm.argtypes = [C.c_char_p, C.c_uint32, u32p, C.c_uint64, u32p, u32p, C.c_uint32,
                                 u32p, u8p, u32p, u32p, u32p]
@@ -62,6 +65,7 @@ def _lib():
 
 class Query:
     def __init__(self, n, labels, edges, shape=""):
+        self.cost = None
         self.n = int(n)
         self.labels = np.asarray(labels, np.uint32)
         self.edges = [(int(a), int(b)) for a, b in edges]
@@ -159,18 +163,26 @@ def shape_edges(kind, k, seed=0):
 
 
 def extract_query(n, off, nbr, labels, kind, k, seed, max_start_deg=1 << 30, max_walk_deg=1 << 30,
-                  budget=200000, attempts=2000):
+                  budget=200000, attempts=2000, min_label=0):
     """Query of the given shape whose vertices are a randomized-walk embedding
-    in the data graph; labels copied from the data vertices it landed on."""
+    in the data graph (through vertices with label >= min_label); labels copied
+    from the data vertices it landed on."""
     edges = shape_edges(kind, k, seed)
     pu = np.array([a for a, _ in edges], np.uint32)
     pv = np.array([b for _, b in edges], np.uint32)
     out = np.zeros(k, np.uint32)
-    rc = _lib().gen_find_pattern(n, off, nbr, k, len(edges), pu, pv, seed, max_start_deg,
-                                 max_walk_deg, budget, attempts, out)
+    rc = _lib().gen_find_pattern(n, off, nbr, np.ascontiguousarray(labels, np.uint32), min_label, k,
+                                 len(edges), pu, pv, seed, max_start_deg, max_walk_deg, budget, attempts,
+                                 out)
     if rc != 0:
         return None
     return Query(k, labels[out], edges, shape="%s%d" % (kind, k))
+
+
+def estimate_per_edge(n, off, nbr, labels, q, probes=4000, seed=1):
+    """Knuth estimate of the embeddings one random edge record seeds (gen.cpp)."""
+    return _lib().gen_estimate_per_edge(n, off, nbr, np.ascontiguousarray(labels, np.uint32), q.n,
+                                        len(q.edges), q.eu, q.ev, q.labels, probes, seed)
 
 
 # -------------------------------------------------------------- workloads
@@ -221,7 +233,7 @@ def small(seed=1):
     off = np.zeros(n + 1, np.uint64)
     nbr = np.zeros(2 * int(keep.sum()), np.uint32)
     L.gen_csr(n, int(keep.sum()), eu[keep].copy(), ev[keep].copy(), off, nbr)
-    q = extract_query(n, off, nbr, labels, "tri_pendant", 4, seed + 30)
+    q = extract_query(n, off, nbr, labels, "tri_pendant", 4, seed + 30, min_label=0)
     assert q is not None
     return _assemble("small", n, labels, eu, ev, keep, recs, 1, [q], seed + 40)
 
@@ -246,8 +258,11 @@ def _rmat_graph(n, m, nl, s, seed):
 
 
 def rmat(n=1000000, m=10000000, nl=16, zipf=1.0, batch=4096, seed=2, queries=("cycle5_chord",),
-         frac=0.10, name="rmat1m", query_specs=None):
-    """R-MAT graph with a held-out insert stream and random deletes."""
+         frac=0.10, name="rmat1m", query_specs=None, max_cost=1.0e4, max_tries=256):
+    """R-MAT graph with a held-out insert stream and random deletes.
+
+    With max_cost, a drawn query whose estimated embeddings per edge record
+    (estimate_per_edge, on the initial graph) exceed it is redrawn."""
     eu, ev, labels = _rmat_graph(n, m, nl, zipf, seed)
     n_ins = int(m * frac)
     keep, recs = _stream(m, n_ins, n_ins, batch, seed + 20, interleave=True)
@@ -260,11 +275,19 @@ def rmat(n=1000000, m=10000000, nl=16, zipf=1.0, batch=4096, seed=2, queries=("c
     qs = []
     for i, (kind, k) in enumerate(specs):
         q = None
-        for t in range(64):
+        for t in range(max_tries):
+            # every 16 failed draws the walk moves to rarer labels
             q = extract_query(n, off, nbr, labels, kind, k, seed * 1000 + i * 64 + t + 30,
-                              max_start_deg=256, max_walk_deg=1024, budget=20000, attempts=200)
-            if q is not None:
+                              max_start_deg=256, max_walk_deg=1024, budget=20000, attempts=200,
+                              min_label=min(nl - 1, t // 16))
+            if q is None:
+                continue
+            if max_cost is None:
                 break
+            q.cost = estimate_per_edge(n, off, nbr, labels, q, seed=seed + i * 64 + t)
+            if q.cost <= max_cost:
+                break
+            q = None
         if q is None:
             raise RuntimeError("no embedding found for %s%d" % (kind, k))
         qs.append(q)
@@ -287,5 +310,8 @@ def q64_specs(seed=7):
 
 
 def rmat_q64(**kw):
+    """64 standing queries; each bounded to an estimated 1,000 embeddings per
+    edge record so that a batch stays enumerable."""
     kw.setdefault("name", "rmat1m_q64")
+    kw.setdefault("max_cost", 1000.0)
     return rmat(query_specs=q64_specs(), **kw)

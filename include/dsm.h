@@ -77,6 +77,8 @@ typedef struct {
   uint32_t blocks_per_cu;    /* 0 = as many as occupancy allows */
   uint32_t donate_min;       /* smallest source range split off to an idle wave (0 = default) */
   uint32_t queue_capacity;   /* donated-task queue slots (0 = default) */
+  uint32_t watchdog_ms;      /* an idle DFS wave that waits longer than this for
+                                work aborts the round with DSM_EDEVICE (0 = 60 s) */
 } dsm_options;
 
 /* Per-batch statistics of the last dsm_apply_batch + dsm_match_batch. */
@@ -90,6 +92,10 @@ typedef struct {
   uint32_t relocated_vertices;     /* adjacency lists moved to a larger slot */
   float ms_apply, ms_filter, ms_seed, ms_dfs_pos, ms_dfs_neg; /* device time */
   uint64_t pool_entries_used, pool_entries_capacity;
+  /* DFS work: source-list entries tested (each a 4 B list read plus an 8 B
+   * candidate-bitmap word) and 4 B reads of the membership binary searches */
+  uint64_t dfs_entries, dfs_search_loads;
+  uint32_t dfs_launches;
 } dsm_stats;
 
 /* ------------------------------------------------------------ engine life */
