@@ -122,7 +122,6 @@ struct dsm_engine {
   DevBuf<DfsCtl> ctl;
   PinnedBuf<DfsCtl> h_ctl;
   DevBuf<Task> queue;
-  DevBuf<uint32_t> ready;
   uint32_t qcap = 0, epoch = 0;
   DevBuf<uint32_t> seed_plan, seed_s, seed_d;
   uint32_t seed_cap = 0;
@@ -443,8 +442,7 @@ int upload_tables(dsm_engine *e) {
   HIPCHK(e->h_ctl.ensure(1));
   e->qcap = e->opt.queue_capacity ? e->opt.queue_capacity : (1u << 20);
   HIPCHK(e->queue.alloc(e->qcap));
-  HIPCHK(e->ready.alloc(e->qcap));
-  HIPCHK(hipMemset(e->ready.p, 0, (size_t)e->qcap * 4));
+  HIPCHK(hipMemset(e->queue.p, 0, (size_t)e->qcap * sizeof(Task)));  // ready = 0: no epoch
   HIPCHK(e->tuple_count.alloc(1));
   if (e->opt.tuple_capacity) {
     e->tuple_cap = e->opt.tuple_capacity;
@@ -508,7 +506,6 @@ int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nr
   da.seed_s = e->seed_s.p;
   da.seed_d = e->seed_d.p;
   da.queue = e->queue.p;
-  da.ready = e->ready.p;
   da.qcap = e->qcap;
   da.epoch = ++e->epoch;
   da.donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
@@ -533,7 +530,7 @@ int check_round(dsm_engine *e, uint64_t *seeds) {
     return e->fail_msg(DSM_EDEVICE, "DFS watchdog: a wave waited more than %u ms for work (%d tasks left)",
                        e->watchdog_ms, c.work_left.v);
   *seeds = c.nseeds.v;
-  e->stats.tasks_donated += c.q_tail.v;
+  e->stats.tasks_donated += std::min(c.q_tail.v, e->qcap);
   e->stats.dfs_entries += c.scanned.v;
   e->stats.dfs_search_loads += c.loads.v;
   e->stats.dfs_launches += 1;

@@ -37,14 +37,18 @@ struct alignas(8) SigReq {
 };
 
 // A unit of DFS work: levels [0, level) are fixed in f; level `level` scans
-// source-list indices [lo, hi).
-struct alignas(16) Task {
+// source-list indices [lo, hi). One 128-byte queue slot each; `ready` holds
+// the launch epoch once the slot is published.
+struct alignas(128) Task {
   uint32_t plan;
   uint32_t level;
   uint32_t lo, hi;
   uint32_t f[kMaxQ];
+  uint32_t ready;
+  uint32_t pad[11];
 };
-constexpr int kTaskWords = sizeof(Task) / 4;  // 20
+static_assert(sizeof(Task) == 128, "task slot layout");
+constexpr int kTaskWords = 4 + kMaxQ;  // payload words (20)
 
 // Device-resident control block of one seed + DFS round. Every counter that
 // waves update or poll concurrently has a 128-byte line of its own, so polling
@@ -57,10 +61,9 @@ struct alignas(128) CtlLine {
 struct DfsCtl {
   CtlLine<uint32_t> nseeds;        // seeds written by the seed kernel
   CtlLine<uint32_t> seed_head;     // next seed to claim
-  CtlLine<uint32_t> q_head;        // next queue slot to claim
-  CtlLine<uint32_t> q_tail;        // next queue slot to fill (= tasks donated)
+  CtlLine<uint32_t> q_head;        // queue tickets taken by waves out of work
+  CtlLine<uint32_t> q_tail;        // queue slots reserved by donors (= tasks donated)
   CtlLine<int32_t> work_left;      // seeds + donated tasks not yet finished
-  CtlLine<uint32_t> idle;          // waves waiting for work
   CtlLine<uint32_t> seed_overflow; // seeds dropped for lack of capacity (must stay 0)
   CtlLine<uint32_t> abort;         // set when an idle wave outlived the watchdog
   CtlLine<unsigned long long> scanned;  // source-list entries tested
@@ -123,7 +126,6 @@ struct DfsArgs {
   const uint32_t *seed_s;
   const uint32_t *seed_d;
   Task *queue;
-  uint32_t *ready;
   uint32_t qcap;
   uint32_t epoch;
   uint32_t donate_min;

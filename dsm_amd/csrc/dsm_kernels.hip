@@ -22,7 +22,7 @@ namespace {
 
 constexpr int kApplyWaves = 4;
 constexpr int kLight = 64;          // dirty vertices up to this degree: one lane each
-constexpr uint32_t kDonateEvery = 16;  // chunks between idle checks
+constexpr uint32_t kDonateEvery = 32;  // chunks between checks for waiting waves
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
@@ -456,7 +456,6 @@ __global__ void k_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tup
       ctl->q_head.v = 0;
       ctl->q_tail.v = 0;
       ctl->work_left.v = 0;
-      ctl->idle.v = 0;
       ctl->seed_overflow.v = 0;
       ctl->abort.v = 0;
       ctl->scanned.v = 0;
@@ -577,10 +576,13 @@ __device__ void emit_tuples(const WaveLds &S, const DfsArgs &a, uint32_t l, uint
 // only when at least donate_min source entries remain. The receiver re-fetches
 // and re-filters the donated source range [mid, end).
 __device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l, uint32_t n) {
-  uint32_t idle = 0;
-  if (lane_id() == 0) idle = ld_relaxed(&a.ctl->idle.v);
-  idle = bcast(idle);
-  if (idle == 0) return;
+  // waves waiting for work = tickets taken - slots reserved
+  uint32_t waiting = 0;
+  if (lane_id() == 0) {
+    const uint32_t h = ld_relaxed(&a.ctl->q_head.v), t = ld_relaxed(&a.ctl->q_tail.v);
+    waiting = (h > t && t < a.qcap) ? h - t : 0u;
+  }
+  if (bcast(waiting) == 0) return;
   uint32_t lv = 0xFFFFFFFFu, mid = 0, keep_cn = 0;
   for (uint32_t k = L0; k <= l; k++) {
     const uint32_t p = S.pos[k], e = S.end[k];
@@ -623,8 +625,8 @@ __device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l
   const uint32_t e = S.end[lv];
   uint32_t slot = 0, ok = 0;
   if (lane_id() == 0) {
-    const uint32_t t = ld_relaxed(&a.ctl->q_tail.v);
-    if (t < a.qcap && cas_u32(&a.ctl->q_tail.v, t, t + 1)) {
+    const uint32_t t = atomicAdd(&a.ctl->q_tail.v, 1u);
+    if (t < a.qcap) {
       ok = 1;
       slot = t;
       atomicAdd(&a.ctl->work_left.v, 1);
@@ -643,7 +645,7 @@ __device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l
   if (lane < (uint32_t)kTaskWords) reinterpret_cast<uint32_t *>(&a.queue[slot])[lane] = word;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   if (lane == 0) {
-    st_release(&a.ready[slot], a.epoch);
+    st_release(&a.queue[slot].ready, a.epoch);
     // the donor keeps source indices below mid; after a split inside the
     // chunk, mid <= pos, so nothing is left to fetch at this level and a later
     // split there donates [mid', mid) only
@@ -660,27 +662,24 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
   const uint32_t lane = lane_id();
   if (lane == 0) S.plan = 0xFFFFFFFFu;
   wave_sync();
-  bool idle = false;
   // a seed table overflow leaves unwritten slots: match nothing, the host
   // reports the error (the table is sized so that it cannot happen)
   if (a.ctl->seed_overflow.v) return;
   const uint32_t nseeds = a.ctl->nseeds.v;
   bool seeds_left = true;
-  uint32_t naps = 0;            // idle polls since the last task (backoff)
-  uint64_t idle_since = 0;      // wall clock when the wave went idle (watchdog)
+  // Donated tasks are handed out by ticket: a wave out of work takes the next
+  // ticket with one atomic add and then waits on the ready flag inside that
+  // queue slot (a 128-byte line of its own), so publishing a task wakes exactly
+  // one wave and idle waves never contend on a shared counter.
+  uint32_t ticket = 0xFFFFFFFFu;
+  uint32_t naps = 0;            // polls of the current ticket (backoff)
+  uint64_t idle_since = 0;      // wall clock when the wave ran out of work (watchdog)
   for (;;) {
-    uint32_t got = 0, slot = 0;
+    // lane 0 claims work; every decision is then broadcast so that all
+    // control flow below stays wave-uniform
+    uint32_t got = 0, slot = 0, took = 0;
     if (lane == 0) {
-      uint32_t h = ld_relaxed(&a.ctl->q_head.v);
-      while (h < ld_relaxed(&a.ctl->q_tail.v)) {
-        if (cas_u32(&a.ctl->q_head.v, h, h + 1)) {
-          got = 1;
-          slot = h;
-          break;
-        }
-        h = ld_relaxed(&a.ctl->q_head.v);
-      }
-      if (!got && seeds_left) {
+      if (seeds_left) {
         const uint32_t i = atomicAdd(&a.ctl->seed_head.v, 1u);
         if (i < nseeds) {
           got = 2;
@@ -689,57 +688,49 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
           seeds_left = false;
         }
       }
+      if (!got) {
+        if (ticket == 0xFFFFFFFFu) {
+          ticket = atomicAdd(&a.ctl->q_head.v, 1u);
+          took = 1;
+        }
+        if (ticket < a.qcap && ld_acquire(&a.queue[ticket].ready) == a.epoch) {
+          got = 1;
+          slot = ticket;
+          ticket = 0xFFFFFFFFu;
+        }
+      }
     }
     got = bcast(got);
     slot = bcast(slot);
+    if (bcast(took)) {
+      naps = 0;
+      idle_since = wall_clock64();
+    }
     if (got == 0) {
-      // idle: poll with exponential backoff; the termination counter and the
-      // abort flag only every fourth poll
-      if (!idle) {
-        if (lane == 0) atomicAdd(&a.ctl->idle.v, 1u);
-        idle = true;
-        naps = 0;
-        idle_since = wall_clock64();
-      }
-      if ((naps & 3) == 0) {
+      // waiting for the ticket's task: exponential backoff; the termination
+      // counter, the abort flag and the watchdog every eighth poll
+      if ((naps & 7) == 7) {
         int32_t left = 0;
-        uint32_t ab = 0;
+        uint32_t ab = 0, expired = 0;
         if (lane == 0) {
           left = ld_relaxed(&a.ctl->work_left.v);
           ab = ld_relaxed(&a.ctl->abort.v);
+          expired = wall_clock64() - idle_since > a.watchdog;
+          if (expired && left > 0) st_release(&a.ctl->abort.v, 1u);
         }
         left = (int32_t)bcast((uint32_t)left);
         ab = bcast(ab);
-        if (left <= 0 || ab) break;
-        if (wall_clock64() - idle_since > a.watchdog) {
-          if (lane == 0) st_release(&a.ctl->abort.v, 1u);
-          break;
-        }
+        expired = bcast(expired);
+        if (left <= 0 || ab || expired) break;
       }
-      const uint32_t reps = naps < 6 ? (1u << naps) : 64u;
+      const uint32_t reps = naps < 4 ? (1u << naps) : 16u;
       for (uint32_t r = 0; r < reps; r++) __builtin_amdgcn_s_sleep(8);
       naps++;
       continue;
     }
-    if (idle) {
-      if (lane == 0) atomicSub(&a.ctl->idle.v, 1u);
-      idle = false;
-    }
     // ---- load the task
     uint32_t plan, L0, lo, hi, fw = 0;
     if (got == 1) {
-      if (lane == 0) {
-        // the donor publishes right after reserving the slot; bounded anyway
-        const uint64_t t0 = wall_clock64();
-        while (ld_acquire(&a.ready[slot]) != a.epoch) {
-          __builtin_amdgcn_s_sleep(1);
-          if (wall_clock64() - t0 > a.watchdog) {
-            st_release(&a.ctl->abort.v, 1u);
-            break;
-          }
-        }
-      }
-      if (bcast(ld_relaxed(&a.ctl->abort.v))) break;
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       const uint32_t word =
           lane < (uint32_t)kTaskWords ? reinterpret_cast<const uint32_t *>(&a.queue[slot])[lane] : 0u;

@@ -232,76 +232,89 @@ int gen_find_pattern(uint32_t n, const uint64_t *off, const uint32_t *nbr, const
   return -1;
 }
 
-// Estimated number of embeddings of a labelled query that use one random data
-// edge (s,d) (uniform over the 2m directed adjacency entries), by Knuth's
-// estimator of backtracking-tree size: each probe draws an edge and an ordered
-// query edge (u,v), maps u->s, v->d, then extends the remaining query vertices
-// in a connected order, picking one valid extension uniformly at random and
-// multiplying the number of valid extensions seen at every level. The mean of
-// the products, times the number of ordered query edges, is an unbiased
-// estimate of the embeddings seeded per edge record. Returns -1 on bad input.
-double gen_estimate_per_edge(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab,
-                             uint32_t k, uint32_t pm, const uint32_t *pu, const uint32_t *pv,
-                             const uint32_t *qlab, uint32_t probes, uint64_t seed) {
-  if (k < 2 || k > 32 || pm == 0 || off[n] == 0) return -1.0;
-  std::vector<uint32_t> adj(k, 0);
-  std::vector<std::pair<uint32_t, uint32_t>> qe;
-  for (uint32_t i = 0; i < pm; i++) {
-    adj[pu[i]] |= 1u << pv[i];
-    adj[pv[i]] |= 1u << pu[i];
-    qe.push_back({pu[i], pv[i]});
-    qe.push_back({pv[i], pu[i]});
+// Pilot of a query's enumeration cost over given edge records: for every record
+// (s,d) and ordered query edge (u,v) with matching labels, counts the nodes of
+// the backtracking tree that extends u->s, v->d (every valid partial embedding,
+// leaves included), with the usual candidate rule (label, adjacency to every
+// mapped query neighbour, injectivity) on the given graph. Stops as soon as
+// the total exceeds `budget` and then returns budget + 1.
+namespace {
+struct Pilot {
+  const uint64_t *off;
+  const uint32_t *nbr;
+  const uint32_t *lab;
+  const uint32_t *qlab;
+  uint32_t k;
+  uint32_t adj[32];
+  uint32_t order[32];
+  uint32_t f[32];
+  uint64_t nodes, budget;
+  bool adjacent(uint32_t a, uint32_t b) const {
+    return std::binary_search(nbr + off[a], nbr + off[a + 1], b);
   }
-  Rng r(seed);
-  double total = 0;
-  std::vector<uint32_t> f(k), cand;
-  for (uint32_t pr = 0; pr < probes; pr++) {
-    const uint64_t ent = r.below(off[n]);
-    const uint32_t s = (uint32_t)(std::upper_bound(off, off + n + 1, ent) - off - 1);
-    const uint32_t d = nbr[ent];
-    const auto &e = qe[r.below(qe.size())];
-    if (lab[s] != qlab[e.first] || lab[d] != qlab[e.second] || s == d) continue;
-    uint32_t placed = (1u << e.first) | (1u << e.second);
-    f[e.first] = s;
-    f[e.second] = d;
-    double prod = 1.0;
-    for (uint32_t step = 2; step < k && prod > 0; step++) {
-      // next: the unplaced vertex with the most placed neighbours
-      int best = -1, bb = -1;
-      for (uint32_t w = 0; w < k; w++) {
-        if (placed >> w & 1) continue;
-        int b = __builtin_popcount(adj[w] & placed);
-        if (b > bb) { bb = b; best = (int)w; }
+  void rec(uint32_t p) {
+    if (p == k || nodes > budget) return;
+    const uint32_t u = order[p];
+    uint32_t src = 0;
+    uint64_t sdeg = ~0ull;
+    for (uint32_t j = 0; j < p; j++)
+      if (adj[u] >> order[j] & 1) {
+        const uint64_t dg = off[f[order[j]] + 1] - off[f[order[j]]];
+        if (dg < sdeg) { sdeg = dg; src = order[j]; }
       }
-      const uint32_t u = (uint32_t)best;
-      // scan the shortest backward list
-      uint32_t src = 0;
-      uint64_t sdeg = ~0ull;
-      for (uint32_t j = 0; j < k; j++)
-        if ((placed >> j & 1) && (adj[u] >> j & 1)) {
-          uint64_t dg = off[f[j] + 1] - off[f[j]];
-          if (dg < sdeg) { sdeg = dg; src = j; }
-        }
-      cand.clear();
-      for (uint64_t i = off[f[src]]; i < off[f[src] + 1]; i++) {
-        const uint32_t w = nbr[i];
-        if (lab[w] != qlab[u]) continue;
-        bool ok = true;
-        for (uint32_t j = 0; j < k && ok; j++) {
-          if (!(placed >> j & 1)) continue;
-          if (f[j] == w) ok = false;
-          else if (j != src && (adj[u] >> j & 1) && !std::binary_search(nbr + off[w], nbr + off[w + 1], f[j])) ok = false;
-        }
-        if (ok) cand.push_back(w);
+    const uint32_t x = f[src];
+    for (uint64_t i = off[x]; i < off[x + 1] && nodes <= budget; i++) {
+      const uint32_t w = nbr[i];
+      if (lab[w] != qlab[u]) continue;
+      bool ok = true;
+      for (uint32_t j = 0; j < p && ok; j++) {
+        const uint32_t qj = order[j];
+        if (f[qj] == w) ok = false;
+        else if (qj != src && (adj[u] >> qj & 1) && !adjacent(w, f[qj])) ok = false;
       }
-      prod *= (double)cand.size();
-      if (cand.empty()) break;
-      f[u] = cand[r.below(cand.size())];
-      placed |= 1u << u;
+      if (!ok) continue;
+      nodes++;
+      f[u] = w;
+      rec(p + 1);
     }
-    total += prod;
   }
-  return total / probes * (double)qe.size();
+};
+}  // namespace
+
+uint64_t gen_pilot_nodes(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab, uint32_t k,
+                         uint32_t pm, const uint32_t *pu, const uint32_t *pv, const uint32_t *qlab, uint64_t nrec,
+                         const uint32_t *rs, const uint32_t *rd, uint64_t budget) {
+  if (k < 2 || k > 32) return budget + 1;
+  Pilot P;
+  P.off = off; P.nbr = nbr; P.lab = lab; P.qlab = qlab; P.k = k; P.nodes = 0; P.budget = budget;
+  for (uint32_t i = 0; i < 32; i++) P.adj[i] = 0;
+  for (uint32_t i = 0; i < pm; i++) { P.adj[pu[i]] |= 1u << pv[i]; P.adj[pv[i]] |= 1u << pu[i]; }
+  for (uint64_t r = 0; r < nrec && P.nodes <= budget; r++) {
+    const uint32_t s = rs[r], d = rd[r];
+    if (s >= n || d >= n || s == d) continue;
+    for (uint32_t u = 0; u < k && P.nodes <= budget; u++) {
+      if (qlab[u] != lab[s]) continue;
+      for (uint32_t v = 0; v < k; v++) {
+        if (!(P.adj[u] >> v & 1) || qlab[v] != lab[d]) continue;
+        uint32_t placed = (1u << u) | (1u << v);
+        P.order[0] = u; P.order[1] = v;
+        for (uint32_t p = 2; p < k; p++) {
+          int best = -1, bb = -1;
+          for (uint32_t w = 0; w < k; w++) {
+            if (placed >> w & 1) continue;
+            int b = __builtin_popcount(P.adj[w] & placed);
+            if (b > bb) { bb = b; best = (int)w; }
+          }
+          P.order[p] = (uint32_t)best;
+          placed |= 1u << best;
+        }
+        P.f[u] = s; P.f[v] = d;
+        P.nodes++;
+        P.rec(2);
+      }
+    }
+  }
+  return P.nodes > budget ? budget + 1 : P.nodes;
 }
 
 // Writes a data graph + update batches (or, with nb == UINT32_MAX, a query

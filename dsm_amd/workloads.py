@@ -53,9 +53,9 @@ def _lib():
     L.gen_find_pattern.restype = C.c_int
     L.gen_find_pattern.argtypes = [C.c_uint32, u64p, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p,
                                    u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, u32p]
-    L.gen_estimate_per_edge.restype = C.c_double
-    L.gen_estimate_per_edge.argtypes = [C.c_uint32, u64p, u32p, u32p, C.c_uint32, C.c_uint32, u32p, u32p,
-                                        u32p, C.c_uint32, C.c_uint64]
+    L.gen_pilot_nodes.restype = C.c_uint64
+    L.gen_pilot_nodes.argtypes = [C.c_uint32, u64p, u32p, u32p, C.c_uint32, C.c_uint32, u32p, u32p, u32p,
+                                  C.c_uint64, u32p, u32p, C.c_uint64]
     L.gen_write_dsm.restype = C.c_int
     L.gen_write_dsm.argtypes = [C.c_char_p, C.c_uint32, u32p, C.c_uint64, u32p, u32p, C.c_uint32,
                                 u32p, u8p, u32p, u32p, u32p]
@@ -66,6 +66,7 @@ def _lib():
 class Query:
     def __init__(self, n, labels, edges, shape=""):
         self.cost = None
+        self.draw = None
         self.n = int(n)
         self.labels = np.asarray(labels, np.uint32)
         self.edges = [(int(a), int(b)) for a, b in edges]
@@ -179,10 +180,12 @@ def extract_query(n, off, nbr, labels, kind, k, seed, max_start_deg=1 << 30, max
     return Query(k, labels[out], edges, shape="%s%d" % (kind, k))
 
 
-def estimate_per_edge(n, off, nbr, labels, q, probes=4000, seed=1):
-    """Knuth estimate of the embeddings one random edge record seeds (gen.cpp)."""
-    return _lib().gen_estimate_per_edge(n, off, nbr, np.ascontiguousarray(labels, np.uint32), q.n,
-                                        len(q.edges), q.eu, q.ev, q.labels, probes, seed)
+def pilot_nodes(n, off, nbr, labels, q, rs, rd, budget):
+    """Backtracking-tree nodes of q over the records (rs, rd) in the graph
+    (off, nbr), stopping at budget + 1 (gen.cpp gen_pilot_nodes)."""
+    return int(_lib().gen_pilot_nodes(n, off, nbr, np.ascontiguousarray(labels, np.uint32), q.n, len(q.edges),
+                                      q.eu, q.ev, q.labels, len(rs), np.ascontiguousarray(rs, np.uint32),
+                                      np.ascontiguousarray(rd, np.uint32), budget))
 
 
 # -------------------------------------------------------------- workloads
@@ -257,41 +260,75 @@ def _rmat_graph(n, m, nl, s, seed):
     return _RMAT_CACHE[key]
 
 
-def rmat(n=1000000, m=10000000, nl=16, zipf=1.0, batch=4096, seed=2, queries=("cycle5_chord",),
-         frac=0.10, name="rmat1m", query_specs=None, max_cost=1.0e4, max_tries=256):
-    """R-MAT graph with a held-out insert stream and random deletes.
+def _pick_query(n, off, nbr, labels, nl, kind, k, seed0, rs, rd, budget, max_tries):
+    for t in range(max_tries):
+        # every 16 redraws the walk moves to rarer labels
+        q = extract_query(n, off, nbr, labels, kind, k, seed0 + t, max_start_deg=256, max_walk_deg=1024,
+                          budget=5000, attempts=50, min_label=min(nl - 1, t // 16))
+        if q is None:
+            continue
+        q.cost = pilot_nodes(n, off, nbr, labels, q, rs, rd, budget)
+        if q.cost <= budget:
+            q.draw = t
+            return q
+    raise RuntimeError("no query of shape %s%d within the budget" % (kind, k))
 
-    With max_cost, a drawn query whose estimated embeddings per edge record
-    (estimate_per_edge, on the initial graph) exceed it is redrawn."""
+
+def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=512, threads=8, log=None):
+    """Draws one query per (shape, size) spec by randomized walk on the initial
+    graph of w. A draw whose backtracking tree over the records of the first
+    `window` batches (pilot on the initial graph, pilot_nodes) has more than
+    `budget` nodes is redrawn; every 16 redraws the walk moves to rarer labels.
+    This keeps the stream enumerable: R-MAT hubs give some edges astronomically
+    many embeddings of trees and long paths. The pilot visits the records with
+    the highest endpoint degrees first, so explosive draws are rejected early."""
+    from concurrent.futures import ThreadPoolExecutor
+    off, nbr = w.csr()
+    hi = int(w.bstart[min(window, w.num_batches)])
+    deg = np.diff(off).astype(np.int64)
+    rs, rd = w.src[:hi], w.dst[:hi]
+    order = np.argsort(-(deg[rs] + deg[rd]), kind="stable")
+    rs, rd = np.ascontiguousarray(rs[order]), np.ascontiguousarray(rd[order])
+
+    def one(i):
+        kind, k = specs[i]
+        q = _pick_query(w.n, off, nbr, w.labels, nl, kind, k, seed * 1000 + i * 1024 + 30, rs, rd, budget,
+                        max_tries)
+        if log:
+            log("query %d: %s labels %s pilot %d nodes (draw %d)" % (i, q.shape, q.labels.tolist(), q.cost,
+                                                                     q.draw))
+        return q
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, range(len(specs))))
+
+
+QUERY_DIR = os.path.join(_HERE, "queries")
+
+
+def load_queries(name):
+    import json
+    with open(os.path.join(QUERY_DIR, name + ".json")) as f:
+        d = json.load(f)
+    qs = []
+    for x in d["queries"]:
+        q = Query(len(x["labels"]), x["labels"], x["edges"], shape=x["shape"])
+        q.cost = x.get("pilot_nodes")
+        qs.append(q)
+    return qs
+
+
+def rmat(n=1000000, m=10000000, nl=16, zipf=1.0, batch=4096, seed=2, frac=0.10, name="rmat1m",
+         queries=None):
+    """R-MAT graph with a held-out insert stream and random deletes; the
+    standing queries are the committed set `queries/<name>.json` (drawn by
+    draw_queries, see queries/draw.py) unless a list is given."""
     eu, ev, labels = _rmat_graph(n, m, nl, zipf, seed)
     n_ins = int(m * frac)
     keep, recs = _stream(m, n_ins, n_ins, batch, seed + 20, interleave=True)
-    L = _lib()
-    ie, iv = eu[keep].copy(), ev[keep].copy()
-    off = np.zeros(n + 1, np.uint64)
-    nbr = np.zeros(2 * len(ie), np.uint32)
-    L.gen_csr(n, len(ie), ie, iv, off, nbr)
-    specs = query_specs if query_specs is not None else [(k, 6 if k == "cycle5_chord" else 4) for k in queries]
-    qs = []
-    for i, (kind, k) in enumerate(specs):
-        q = None
-        for t in range(max_tries):
-            # every 16 failed draws the walk moves to rarer labels
-            q = extract_query(n, off, nbr, labels, kind, k, seed * 1000 + i * 64 + t + 30,
-                              max_start_deg=256, max_walk_deg=1024, budget=20000, attempts=200,
-                              min_label=min(nl - 1, t // 16))
-            if q is None:
-                continue
-            if max_cost is None:
-                break
-            q.cost = estimate_per_edge(n, off, nbr, labels, q, seed=seed + i * 64 + t)
-            if q.cost <= max_cost:
-                break
-            q = None
-        if q is None:
-            raise RuntimeError("no embedding found for %s%d" % (kind, k))
-        qs.append(q)
-    return _assemble(name, n, labels, eu, ev, keep, recs, batch, qs, seed + 40)
+    w = _assemble(name, n, labels, eu, ev, keep, recs, batch, [], seed + 40)
+    w.queries = list(queries) if queries is not None else load_queries(name)
+    return w
 
 
 def q64_specs(seed=7):
@@ -310,8 +347,6 @@ def q64_specs(seed=7):
 
 
 def rmat_q64(**kw):
-    """64 standing queries; each bounded to an estimated 1,000 embeddings per
-    edge record so that a batch stays enumerable."""
+    """The same graph and stream with the 64 committed standing queries."""
     kw.setdefault("name", "rmat1m_q64")
-    kw.setdefault("max_cost", 1000.0)
-    return rmat(query_specs=q64_specs(), **kw)
+    return rmat(**kw)

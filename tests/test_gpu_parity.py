@@ -260,9 +260,10 @@ def test_rmat_scaled_stream():
     """R-MAT (.57,.19,.19) 20,000 V / 200,000 E, 16 Zipf labels, random-walk
     queries of mixed shapes, batches of 1,024 mixed records."""
     say("case: scaled R-MAT stream")
-    w = W.rmat(n=20000, m=200000, batch=1024, seed=9, name="rmat20k",
-               query_specs=[("cycle5_chord", 6), ("clique4", 4), ("tree", 5), ("cycle", 5), ("chord_cycle", 6),
-                            ("tree", 4)])
+    w = W.rmat(n=20000, m=200000, batch=1024, seed=9, name="rmat20k", queries=[])
+    w.queries = W.draw_queries(w, [("cycle5_chord", 6), ("clique4", 4), ("tree", 5), ("cycle", 5),
+                                   ("chord_cycle", 6), ("tree", 4)], seed=9, nl=16, budget=3_000_000,
+                               window=3)
     queries = [(q.n, q.labels, q.eu, q.ev) for q in w.queries]
     compare_stream(w.n, w.labels, w.eu, w.ev, queries, w.batches(0, 3), relabel=True, check_graph=False)
 
