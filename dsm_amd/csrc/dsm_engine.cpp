@@ -131,7 +131,7 @@ struct dsm_engine {
   uint32_t *tuples = nullptr;
   bool tuples_host = false;
   unsigned long long tuple_cap = 0;
-  uint32_t dfs_blocks = 0, dfs_wpb = 4;
+  uint32_t dfs_blocks = 0, dfs_wpb = 4, max_qn = 2;
   uint32_t watchdog_ms = 60000;
   int wall_khz = 100000;
   // records of the last batch (file order)
@@ -516,7 +516,7 @@ int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nr
   da.tuples = e->tuples;
   da.tuple_cap = e->tuple_cap;
   da.tuple_count = e->tuple_count.p;
-  HIPCHK(launch_dfs(da, e->dfs_blocks, e->dfs_wpb, e->stream));
+  HIPCHK(launch_dfs(da, e->dfs_blocks, e->dfs_wpb, e->max_qn, e->stream));
   HIPCHK(hipEventRecord(ev2, e->stream));
   HIPCHK(hipMemcpyAsync(e->h_ctl.p, e->ctl.p, sizeof(DfsCtl), hipMemcpyDeviceToHost, e->stream));
   return DSM_OK;
@@ -759,7 +759,12 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   uint32_t wpb = e->opt.waves_per_block;
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) wpb = 4;
   e->dfs_wpb = wpb;
-  int per_cu = dfs_max_blocks_per_cu(wpb);
+  e->max_qn = 2;
+  for (auto &q : e->queries) e->max_qn = std::max(e->max_qn, q.n);
+  // keep a block's LDS (waves x per-wave stack) within 64 KB
+  while (wpb > 1 && wpb * dfs_lds_bytes_per_wave(e->max_qn) > 65536) wpb /= 2;
+  e->dfs_wpb = wpb;
+  int per_cu = dfs_max_blocks_per_cu(wpb, e->max_qn);
   if (e->opt.blocks_per_cu && (int)e->opt.blocks_per_cu < per_cu) per_cu = (int)e->opt.blocks_per_cu;
   e->dfs_blocks = (uint32_t)(per_cu * e->num_cu);
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));

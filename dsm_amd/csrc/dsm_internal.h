@@ -178,7 +178,8 @@ hipError_t launch_filter_full(const DevGraph &g, const FilterTables &ft, hipStre
 hipError_t launch_filter_dirty(const DevGraph &g, const FilterTables &ft, const DirtyArgs &d,
                                hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
-hipError_t launch_dfs(const DfsArgs &a, uint32_t blocks, uint32_t waves_per_block, hipStream_t s);
+hipError_t launch_dfs(const DfsArgs &a, uint32_t blocks, uint32_t waves_per_block, uint32_t max_query_vertices,
+                      hipStream_t s);
 // Resets the DFS control block (ctl) and/or the per-batch state (actl's
 // counters, the tuple counter and the nq*2 match counts); null pointers are skipped.
 hipError_t launch_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tuple_count,
@@ -187,6 +188,7 @@ hipError_t launch_scratch_reset(ApplyCtl *actl, hipStream_t s);
 hipError_t launch_pool_move(const uint32_t *old_pool, uint32_t *new_pool, const uint32_t *old_start,
                             const uint32_t *new_start, const uint32_t *deg, uint32_t n,
                             hipStream_t s);
-int dfs_max_blocks_per_cu(uint32_t waves_per_block);
+int dfs_max_blocks_per_cu(uint32_t waves_per_block, uint32_t max_query_vertices);
+uint32_t dfs_lds_bytes_per_wave(uint32_t max_query_vertices);
 
 }  // namespace dsm

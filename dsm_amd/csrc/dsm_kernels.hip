@@ -46,10 +46,6 @@ __device__ __forceinline__ T ld_relaxed(T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
-__device__ __forceinline__ T ld_acquire(T *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
 __device__ __forceinline__ void st_release(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -478,8 +474,11 @@ __global__ void k_scratch_reset(ApplyCtl *actl) {
 
 // -------------------------------------------------------------------- DFS
 
-struct WaveLds {
-  uint32_t cand[kMaxQ][64];  // current candidate chunk of each level
+// Per-wave DFS state, in dynamic LDS: a fixed header, a window buffer for the
+// cooperative intersections and, for each chunk level 2 .. n-2, 64 candidates
+// with their prefetched list start and degree. Sized to the engine's largest
+// query, so small queries keep high occupancy.
+struct WaveHdr {
   uint32_t f[kMaxQ];         // data vertex of each position
   uint32_t fs[kMaxQ];        // its list start
   uint32_t fd[kMaxQ];        // its degree
@@ -494,19 +493,23 @@ struct WaveLds {
   uint16_t back[kMaxQ];
   uint16_t inj[kMaxQ];
   uint8_t qv[kMaxQ];
-  uint32_t n, query, plan;
+  uint32_t n, query, plan, pad;
+};
+constexpr uint32_t kWin = 256;  // window entries staged in LDS per intersection
+
+struct WaveMem {
+  WaveHdr *h;
+  uint32_t *win;    // [kWin]
+  uint32_t *cand;   // [levels][64]
+  uint32_t *cst;    // [levels][64] list start of each candidate
+  uint32_t *cdg;    // [levels][64] degree of each candidate
 };
 
-__device__ __forceinline__ void assign(WaveLds &S, const DevGraph &g, uint32_t l, uint32_t w) {
-  if (lane_id() == 0) {
-    S.f[l] = w;
-    S.fs[l] = g.start[w];
-    S.fd[l] = g.deg[w];
-  }
-  wave_sync();
+__host__ __device__ constexpr uint32_t wave_lds_bytes(uint32_t chunk_levels) {
+  return (uint32_t)((sizeof(WaveHdr) + 15) / 16 * 16) + kWin * 4 + chunk_levels * 64 * 12;
 }
 
-__device__ __forceinline__ void init_level(WaveLds &S, uint32_t l, uint32_t lo, uint32_t hi) {
+__device__ __forceinline__ void init_level(WaveHdr &S, uint32_t l, uint32_t lo, uint32_t hi) {
   const uint32_t bm = S.back[l];
   uint32_t best = bm ? 0xFFFFFFFFu : 0u, bj = 0;  // no backward neighbour: empty scan
   for (uint32_t j = 0; j < l; j++)
@@ -525,32 +528,103 @@ __device__ __forceinline__ void init_level(WaveLds &S, uint32_t l, uint32_t lo, 
   wave_sync();
 }
 
-// Is w a valid image for position l: candidate bit, injectivity against
-// earlier positions of the same label, adjacency to every backward neighbour
-// other than the one whose list produced w.
-// `loads` counts the 4-byte list reads of the membership searches (for the
-// algorithmic-bytes accounting of the roofline).
-__device__ __forceinline__ bool check(const WaveLds &S, const DfsArgs &a, uint32_t l, uint32_t w,
-                                      uint32_t &loads) {
-  if (!bit_of(a.bm, a.words, S.sig[l], w)) return false;
+// Lower bounds of key0 (lanes 0-31) and key1 (lanes 32-63) in sorted L[0, d),
+// by a 32-ary search per half-wave: every round probes 32 evenly spaced
+// entries of the remaining range (one load per lane) and keeps the segment
+// the key falls in, so a list of d entries takes about log32(d) + 1 dependent
+// loads instead of log2(d).
+__device__ __forceinline__ void bounds2(const uint32_t *L, uint32_t d, uint32_t key0, uint32_t key1,
+                                        uint32_t &r0, uint32_t &r1, uint32_t &loads) {
+  const uint32_t lane = lane_id();
+  const uint32_t k = lane & 31;
+  const uint32_t key = lane < 32 ? key0 : key1;
+  const uint64_t halfmask = lane < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
+  uint32_t a = 0, b = d;  // the bound lies in [a, b]
+  while (ballot(b > a)) {
+    const uint32_t m = b - a;
+    const uint32_t step = m > 32 ? (m + 31) / 32 : 1u;
+    const uint32_t p = a + k * step;
+    bool less = false;
+    if (p < b) {
+      less = L[p] < key;
+      loads++;
+    }
+    const uint32_t c = (uint32_t)__popcll(ballot(less) & halfmask);
+    if (b > a) {
+      if (c == 0) {
+        b = a;
+      } else {
+        const uint32_t na = a + (c - 1) * step + 1;
+        const uint32_t nb = a + c * step < b ? a + c * step : b;
+        a = na;
+        b = nb;
+      }
+    }
+  }
+  r0 = __builtin_amdgcn_readlane(a, 0);
+  r1 = __builtin_amdgcn_readlane(a, 32);
+}
+
+__device__ __forceinline__ bool lds_contains(const uint32_t *a, uint32_t n, uint32_t x) {
+  if (n == 0) return false;
+  uint32_t lo = 0;
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+    lo = (a[lo + half] <= x) ? lo + half : lo;
+    n -= half;
+  }
+  return a[lo] == x;
+}
+
+// Tests source entries [p, min(p + 64, e)) of level l, one per lane: the
+// candidate bit of the level's signature, injectivity against earlier
+// positions of the same label, then membership in every other backward list.
+// The source chunk is sorted, so the entries still alive span [wmin, wmax];
+// the wave finds that window of the other list cooperatively (bounds2) and,
+// when it holds at most kWin entries, stages it in LDS for the lanes' searches.
+__device__ __forceinline__ bool test_chunk(const WaveMem &M, const DfsArgs &a, uint32_t l, uint32_t p,
+                                           uint32_t e, uint32_t &w, uint32_t &loads) {
+  WaveHdr &S = *M.h;
+  const uint32_t lane = lane_id();
+  const uint32_t i = p + lane;
+  const uint32_t sj = S.srcj[l];
+  bool ok = i < e;
+  w = ok ? a.g.pool[S.fs[sj] + i] : 0xFFFFFFFFu;
+  ok = ok && bit_of(a.bm, a.words, S.sig[l], w);
   uint32_t inj = S.inj[l];
   while (inj) {
     const uint32_t j = __builtin_ctz(inj);
     inj &= inj - 1;
-    if (S.f[j] == w) return false;
+    ok = ok && S.f[j] != w;
   }
-  uint32_t oth = S.back[l] & ~(1u << S.srcj[l]);
+  uint32_t oth = S.back[l] & ~(1u << sj);
   while (oth) {
     const uint32_t j = __builtin_ctz(oth);
     oth &= oth - 1;
-    const uint32_t d = S.fd[j];
-    loads += d ? 32u - (uint32_t)__builtin_clz(d) : 0u;
-    if (!contains(a.g.pool + S.fs[j], d, w)) return false;
+    const uint64_t alive = ballot(ok);
+    if (alive == 0) break;
+    const uint32_t wmin = __shfl(w, (int)__builtin_ctzll(alive));
+    const uint32_t wmax = __shfl(w, (int)(63 - __builtin_clzll(alive)));
+    const uint32_t *L = a.g.pool + S.fs[j];
+    uint32_t lo, hi;
+    bounds2(L, S.fd[j], wmin, wmax + 1, lo, hi, loads);
+    const uint32_t W = hi - lo;
+    if (W <= kWin) {
+      for (uint32_t k = lane; k < W; k += 64) M.win[k] = L[lo + k];
+      wave_sync();
+      ok = ok && lds_contains(M.win, W, w);
+      wave_sync();
+    } else {
+      if (ok) {
+        loads += 32u - (uint32_t)__builtin_clz(W);
+        ok = contains(L + lo, W, w);
+      }
+    }
   }
-  return true;
+  return ok;
 }
 
-__device__ void emit_tuples(const WaveLds &S, const DfsArgs &a, uint32_t l, uint32_t w, bool ok) {
+__device__ void emit_tuples(const WaveHdr &S, const DfsArgs &a, uint32_t l, uint32_t w, bool ok) {
   const uint64_t m = ballot(ok);
   if (m == 0) return;
   unsigned long long base = 0;
@@ -567,15 +641,15 @@ __device__ void emit_tuples(const WaveLds &S, const DfsArgs &a, uint32_t l, uint
   t[2 + S.qv[l]] = w;
 }
 
-// Hands half of the shallowest large-enough remaining source range to the
-// queue when some wave is idle.
+// Hands half of some remaining work to the queue when waves are waiting.
 //
-// Levels shallower than the current one give away work when at least two
-// candidates remain there (fetched-but-unconsumed chunk entries count: their
-// source indices are recovered from the chunk mask); the level being scanned
-// only when at least donate_min source entries remain. The receiver re-fetches
-// and re-filters the donated source range [mid, end).
-__device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l, uint32_t n) {
+// Levels shallower than the current one give away work when enough entries
+// remain there (fetched-but-unconsumed chunk entries count: their source
+// indices are recovered from the chunk mask): two, when the level is two or
+// more above the leaf; sixteen one level above it. The level being scanned
+// gives away work only when at least donate_min source entries remain. The
+// receiver re-fetches and re-filters the donated source range [mid, end).
+__device__ void try_donate(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l, uint32_t n) {
   // waves waiting for work = tickets taken - slots reserved
   uint32_t waiting = 0;
   if (lane_id() == 0) {
@@ -621,7 +695,6 @@ __device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l
     }
   }
   if (lv == 0xFFFFFFFFu) return;
-  (void)n;
   const uint32_t e = S.end[lv];
   uint32_t slot = 0, ok = 0;
   if (lane_id() == 0) {
@@ -656,10 +729,19 @@ __device__ void try_donate(WaveLds &S, const DfsArgs &a, uint32_t L0, uint32_t l
 }
 
 template <int WPB, bool TUP>
-__global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
-  __shared__ WaveLds s_all[WPB];
-  WaveLds &S = s_all[threadIdx.x >> 6];
+__global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_levels) {
+  extern __shared__ __align__(16) uint8_t s_dyn[];
   const uint32_t lane = lane_id();
+  WaveMem M;
+  {
+    uint8_t *base = s_dyn + (threadIdx.x >> 6) * wave_lds_bytes(chunk_levels);
+    M.h = reinterpret_cast<WaveHdr *>(base);
+    M.win = reinterpret_cast<uint32_t *>(base + (sizeof(WaveHdr) + 15) / 16 * 16);
+    M.cand = M.win + kWin;
+    M.cst = M.cand + chunk_levels * 64;
+    M.cdg = M.cst + chunk_levels * 64;
+  }
+  WaveHdr &S = *M.h;
   if (lane == 0) S.plan = 0xFFFFFFFFu;
   wave_sync();
   // a seed table overflow leaves unwritten slots: match nothing, the host
@@ -693,7 +775,9 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
           ticket = atomicAdd(&a.ctl->q_head.v, 1u);
           took = 1;
         }
-        if (ticket < a.qcap && ld_acquire(&a.queue[ticket].ready) == a.epoch) {
+        // relaxed poll: an acquire here would invalidate the CU's L1 on every
+        // poll of every waiting wave; the acquire fence follows a hit instead
+        if (ticket < a.qcap && ld_relaxed(&a.queue[ticket].ready) == a.epoch) {
           got = 1;
           slot = ticket;
           ticket = 0xFFFFFFFFu;
@@ -787,14 +871,9 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
             }
             const uint32_t p = bcast(S.pos[l]), e = bcast(S.end[l]);
             if (p >= e) break;
-            const uint32_t i = p + lane;
-            bool ok = false;
-            uint32_t w = 0;
-            if (i < e) {
-              w = a.g.pool[S.fs[S.srcj[l]] + i];
-              ok = check(S, a, l, w, loads);
-              scanned++;
-            }
+            uint32_t w;
+            const bool ok = test_chunk(M, a, l, p, e, w, loads);
+            scanned += p + lane < e ? 1u : 0u;
             lcount += ok ? 1u : 0u;
             if (TUP) emit_tuples(S, a, l, w, ok);
             wave_sync();
@@ -807,10 +886,18 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
         }
         const uint32_t ci = bcast(S.ci[l]), cn = bcast(S.cn[l]);
         if (ci < cn) {
-          const uint32_t w = bcast(S.cand[l][ci]);
+          const uint32_t c = (l - 2) * 64 + ci;
+          const uint32_t w = bcast(M.cand[c]);
+          const uint32_t ws = bcast(M.cst[c]);
+          const uint32_t wd = bcast(M.cdg[c]);
           wave_sync();
-          if (lane == 0) S.ci[l] = ci + 1;
-          assign(S, a.g, l, w);
+          if (lane == 0) {
+            S.ci[l] = ci + 1;
+            S.f[l] = w;
+            S.fs[l] = ws;
+            S.fd[l] = wd;
+          }
+          wave_sync();
           l++;
           init_level(S, l, 0, 0xFFFFFFFFu);
           continue;
@@ -821,17 +908,23 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a) {
         }
         const uint32_t p = bcast(S.pos[l]), e = bcast(S.end[l]);
         if (p < e) {
-          const uint32_t i = p + lane;
-          bool ok = false;
-          uint32_t w = 0;
-          if (i < e) {
-            w = a.g.pool[S.fs[S.srcj[l]] + i];
-            ok = check(S, a, l, w, loads);
-            scanned++;
+          uint32_t w;
+          const bool ok = test_chunk(M, a, l, p, e, w, loads);
+          scanned += p + lane < e ? 1u : 0u;
+          // prefetch the list of every survivor for the descent
+          uint32_t ws = 0, wd = 0;
+          if (ok) {
+            ws = a.g.start[w];
+            wd = a.g.deg[w];
           }
           const uint64_t m = ballot(ok);
           wave_sync();
-          if (ok) S.cand[l][mbcnt(m)] = w;
+          if (ok) {
+            const uint32_t c = (l - 2) * 64 + mbcnt(m);
+            M.cand[c] = w;
+            M.cst[c] = ws;
+            M.cdg[c] = wd;
+          }
           if (lane == 0) {
             S.cn[l] = __popcll(m);
             S.ci[l] = 0;
@@ -877,9 +970,10 @@ __global__ __launch_bounds__(256) void k_pool_move(const uint32_t *old_pool, uin
 }
 
 template <int WPB>
-int max_blocks() {
+int max_blocks(uint32_t chunk_levels) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_dfs<WPB, false>, WPB * 64, 0) != hipSuccess) return 1;
+  const size_t lds = (size_t)WPB * wave_lds_bytes(chunk_levels);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_dfs<WPB, false>, WPB * 64, lds) != hipSuccess) return 1;
   return nb > 0 ? nb : 1;
 }
 
@@ -947,26 +1041,34 @@ hipError_t launch_seed(const SeedArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_dfs(const DfsArgs &a, uint32_t blocks, uint32_t waves_per_block, hipStream_t s) {
+hipError_t launch_dfs(const DfsArgs &a, uint32_t blocks, uint32_t waves_per_block, uint32_t max_query_vertices,
+                      hipStream_t s) {
+  const uint32_t cl = max_query_vertices > 3 ? max_query_vertices - 3 : 1;
+  const size_t per = wave_lds_bytes(cl);
   if (a.tuples) {
-    hipLaunchKernelGGL((k_dfs<4, true>), dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_dfs<4, true>), dim3(blocks), dim3(256), 4 * per, s, a, cl);
     return hipGetLastError();
   }
   switch (waves_per_block) {
-    case 1: hipLaunchKernelGGL((k_dfs<1, false>), dim3(blocks), dim3(64), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_dfs<2, false>), dim3(blocks), dim3(128), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((k_dfs<8, false>), dim3(blocks), dim3(512), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_dfs<4, false>), dim3(blocks), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_dfs<1, false>), dim3(blocks), dim3(64), per, s, a, cl); break;
+    case 2: hipLaunchKernelGGL((k_dfs<2, false>), dim3(blocks), dim3(128), 2 * per, s, a, cl); break;
+    case 8: hipLaunchKernelGGL((k_dfs<8, false>), dim3(blocks), dim3(512), 8 * per, s, a, cl); break;
+    default: hipLaunchKernelGGL((k_dfs<4, false>), dim3(blocks), dim3(256), 4 * per, s, a, cl); break;
   }
   return hipGetLastError();
 }
 
-int dfs_max_blocks_per_cu(uint32_t waves_per_block) {
+uint32_t dfs_lds_bytes_per_wave(uint32_t max_query_vertices) {
+  return wave_lds_bytes(max_query_vertices > 3 ? max_query_vertices - 3 : 1);
+}
+
+int dfs_max_blocks_per_cu(uint32_t waves_per_block, uint32_t max_query_vertices) {
+  const uint32_t cl = max_query_vertices > 3 ? max_query_vertices - 3 : 1;
   switch (waves_per_block) {
-    case 1: return max_blocks<1>();
-    case 2: return max_blocks<2>();
-    case 8: return max_blocks<8>();
-    default: return max_blocks<4>();
+    case 1: return max_blocks<1>(cl);
+    case 2: return max_blocks<2>(cl);
+    case 8: return max_blocks<8>(cl);
+    default: return max_blocks<4>(cl);
   }
 }
 

@@ -234,10 +234,11 @@ int gen_find_pattern(uint32_t n, const uint64_t *off, const uint32_t *nbr, const
 
 // Pilot of a query's enumeration cost over given edge records: for every record
 // (s,d) and ordered query edge (u,v) with matching labels, counts the nodes of
-// the backtracking tree that extends u->s, v->d (every valid partial embedding,
-// leaves included), with the usual candidate rule (label, adjacency to every
-// mapped query neighbour, injectivity) on the given graph. Stops as soon as
-// the total exceeds `budget` and then returns budget + 1.
+// the backtracking tree that extends u->s, v->d (every valid partial embedding
+// above the last level, plus one per 64 source entries scanned at the last
+// level), with the usual candidate rule (label, adjacency to every mapped query
+// neighbour, injectivity) on the given graph. Stops as soon as the total
+// exceeds `budget` and then returns budget + 1.
 namespace {
 struct Pilot {
   const uint64_t *off;
@@ -263,6 +264,12 @@ struct Pilot {
         if (dg < sdeg) { sdeg = dg; src = order[j]; }
       }
     const uint32_t x = f[src];
+    if (p + 1 == k) {
+      // the last level is counted in 64-entry scan chunks, as a wave64
+      // enumerator counts leaves with one ballot per chunk
+      nodes += (off[x + 1] - off[x] + 63) / 64;
+      return;
+    }
     for (uint64_t i = off[x]; i < off[x + 1] && nodes <= budget; i++) {
       const uint32_t w = nbr[i];
       if (lab[w] != qlab[u]) continue;

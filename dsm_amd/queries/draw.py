@@ -3,8 +3,9 @@
   rmat1m.json      one 6-vertex query (5-cycle with the chord 0-2 plus a
                    pendant vertex), pilot budget 5e7 nodes
   rmat1m_q64.json  64 queries of 4-12 vertices (trees, cycles, 4-cliques,
-                   chorded cycles; shapes from workloads.q64_specs), pilot
-                   budget 5e6 nodes each
+                   chorded cycles; shapes from workloads.q64_specs, shrunk by
+                   a vertex at a time when no draw fits), pilot budget 2e7
+                   nodes each
 
 Both over the records of the first 12 batches of the R-MAT 1M / 10M stream
 (see workloads.draw_queries for the rule). Run from the repository root:
@@ -19,7 +20,7 @@ from dsm_amd import workloads as W
 
 SETS = {
     "rmat1m": {"specs": [("cycle5_chord", 6)], "budget": 50_000_000},
-    "rmat1m_q64": {"specs": W.q64_specs(), "budget": 5_000_000},
+    "rmat1m_q64": {"specs": W.q64_specs(), "budget": 20_000_000},
 }
 
 
@@ -31,7 +32,8 @@ def main(names):
         qs = W.draw_queries(w, cfg["specs"], seed=2, nl=16, budget=cfg["budget"],
                             log=lambda m: print(m, flush=True))
         out = {"workload": name, "rule": "randomized walk; redrawn while the pilot over the first 12 batches "
-                                          "exceeds the budget; rarer labels every 16 redraws",
+                                          "exceeds the budget; rarer labels every 8 redraws; one vertex smaller "
+                                          "after 160 draws",
                "budget_nodes": cfg["budget"], "window_batches": 12,
                "queries": [{"shape": q.shape, "labels": q.labels.tolist(), "edges": q.edges,
                             "pilot_nodes": q.cost, "draw": q.draw} for q in qs]}

@@ -260,25 +260,33 @@ def _rmat_graph(n, m, nl, s, seed):
     return _RMAT_CACHE[key]
 
 
+MIN_SIZE = {"tree": 3, "cycle": 3, "chord_cycle": 4, "clique4": 4, "cycle5_chord": 6, "tri_pendant": 4,
+            "cycle5_2chords": 5}
+
+
 def _pick_query(n, off, nbr, labels, nl, kind, k, seed0, rs, rd, budget, max_tries):
-    for t in range(max_tries):
-        # every 16 redraws the walk moves to rarer labels
-        q = extract_query(n, off, nbr, labels, kind, k, seed0 + t, max_start_deg=256, max_walk_deg=1024,
-                          budget=5000, attempts=50, min_label=min(nl - 1, t // 16))
-        if q is None:
-            continue
-        q.cost = pilot_nodes(n, off, nbr, labels, q, rs, rd, budget)
-        if q.cost <= budget:
-            q.draw = t
-            return q
-    raise RuntimeError("no query of shape %s%d within the budget" % (kind, k))
+    """Draws until the pilot fits the budget; every 8 redraws the walk moves
+    to rarer labels; after max_tries draws the shape shrinks by one vertex."""
+    while k >= MIN_SIZE.get(kind, 3):
+        for t in range(max_tries):
+            q = extract_query(n, off, nbr, labels, kind, k, seed0 + k * 4096 + t, max_start_deg=256,
+                              max_walk_deg=1024, budget=5000, attempts=50, min_label=min(nl - 1, t // 8))
+            if q is None:
+                continue
+            q.cost = pilot_nodes(n, off, nbr, labels, q, rs, rd, budget)
+            if q.cost <= budget:
+                q.draw = t
+                return q
+        k -= 1
+    raise RuntimeError("no query of shape %s within the budget" % kind)
 
 
-def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=512, threads=8, log=None):
+def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=160, threads=8, log=None):
     """Draws one query per (shape, size) spec by randomized walk on the initial
     graph of w. A draw whose backtracking tree over the records of the first
     `window` batches (pilot on the initial graph, pilot_nodes) has more than
-    `budget` nodes is redrawn; every 16 redraws the walk moves to rarer labels.
+    `budget` nodes is redrawn; every 8 redraws the walk moves to rarer labels,
+    and after max_tries draws the shape shrinks by one vertex.
     This keeps the stream enumerable: R-MAT hubs give some edges astronomically
     many embeddings of trees and long paths. The pilot visits the records with
     the highest endpoint degrees first, so explosive draws are rejected early."""
@@ -292,8 +300,18 @@ def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=512, threads=8
 
     def one(i):
         kind, k = specs[i]
-        q = _pick_query(w.n, off, nbr, w.labels, nl, kind, k, seed * 1000 + i * 1024 + 30, rs, rd, budget,
-                        max_tries)
+        q = None
+        # a shape with no draw within the budget falls back to a cycle, then a
+        # tree, of the same size (recorded in q.shape)
+        for alt in (kind, "cycle", "tree"):
+            try:
+                q = _pick_query(w.n, off, nbr, w.labels, nl, alt, k, seed * 1000 + i * 1024 + 30, rs, rd,
+                                budget, max_tries)
+                break
+            except RuntimeError:
+                continue
+        if q is None:
+            raise RuntimeError("no query for spec %d (%s%d) within the budget" % (i, kind, k))
         if log:
             log("query %d: %s labels %s pilot %d nodes (draw %d)" % (i, q.shape, q.labels.tolist(), q.cost,
                                                                      q.draw))
