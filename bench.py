@@ -163,6 +163,23 @@ def run_cpu(w, batch_index, budget_s):
                                                  round(frac_pos * len(oq)), len(oq))}
 
 
+def calibrate_copy(nbytes):
+    """One device-to-device copy of nbytes (the HIP runtime's copy kernel), the
+    known-size reference for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration_copy_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,7 +192,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--waves-per-block", type=int, default=4)
     ap.add_argument("--donate-min", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, help="only run one D2D copy of this many bytes")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine is single-GPU (multi-GPU sharding is design-only); "
               "run with --gpus 1", file=sys.stderr)
