@@ -219,8 +219,15 @@ def main():
     dfs_ms_avg = agg["ms_dfs"] / max(agg["launches"], 1)
     bytes_per_launch = dfs_bytes / max(agg["launches"], 1)
     achieved = bytes_per_launch / (dfs_ms_avg * 1e-3) / 1e9 if dfs_ms_avg > 0 else 0.0
+    # HBM bytes per launch measured by rocprofv3 (FETCH_SIZE / WRITE_SIZE runs
+    # scaled by a copy of known size), kept in profiles/
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_dfs_%s_traffic.json" % w.name)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": "k_dfs", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(dfs_ms_avg, 4)}
 
     out = {"metric": "edge updates/s + p50/p99 batch latency, served/batched vs plain GPU path",
