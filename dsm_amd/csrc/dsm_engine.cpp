@@ -99,6 +99,8 @@ struct dsm_engine {
   // device graph
   DevBuf<uint32_t> pool, start, deg, cap;
   DevBuf<uint8_t> lab;
+  DevBuf<uint32_t> lpool, loff;  // label-sorted copies of long lists
+  uint32_t nbuckets = 1;
   unsigned long long pool_cap = 0;
   // filter
   DevBuf<uint64_t> bm;
@@ -169,6 +171,9 @@ struct dsm_engine {
     g.deg = deg.p;
     g.cap = cap.p;
     g.lab = lab.p;
+    g.lpool = lpool.p;
+    g.loff = loff.p;
+    g.nbuckets = nbuckets;
     return g;
   }
   FilterTables tables() {
@@ -333,6 +338,7 @@ int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq) {
           }
           P.back[k] = back;
           P.inj[k] = inj;
+          P.lix[k] = e->lmap[Q.lab[order[k]]];
         }
         pk.push_back({(Q.lab[u] << 8) | Q.lab[v], P});
       }
@@ -398,6 +404,11 @@ int upload_graph(dsm_engine *e, const dsm_graph_desc *g) {
   HIPCHK(e->deg.alloc(n));
   HIPCHK(e->cap.alloc(n));
   HIPCHK(e->lab.alloc(n));
+  HIPCHK(e->lpool.alloc(pcap));
+  e->nbuckets = 1;
+  for (int l = 0; l < 255; l++)
+    if (e->lmap[l] != 0xFF) e->nbuckets = std::max<uint32_t>(e->nbuckets, e->lmap[l] + 2u);
+  HIPCHK(e->loff.alloc((size_t)n * (e->nbuckets + 1)));
   e->pool_cap = pcap;
   HIPCHK(hipMemcpy(e->pool.p, hp.data(), total * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(e->start.p, st.data(), (size_t)n * 4, hipMemcpyHostToDevice));
@@ -671,6 +682,8 @@ int grow_pool(dsm_engine *e, unsigned long long extra) {
   std::swap(e->start.p, nst.p);
   std::swap(e->start.n, nst.n);
   e->pool_cap = ncap;
+  HIPCHK(e->lpool.alloc(ncap));
+  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->stream));
   // scratch must hold one copy of every list
   if (e->scratch_cap < ncap + (1ull << 20)) {
     e->scratch_cap = ncap + (1ull << 20);
@@ -768,6 +781,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   if (e->opt.blocks_per_cu && (int)e->opt.blocks_per_cu < per_cu) per_cu = (int)e->opt.blocks_per_cu;
   e->dfs_blocks = (uint32_t)(per_cu * e->num_cu);
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
+  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   return DSM_OK;
 }
@@ -874,6 +888,9 @@ int dsm_apply_batch(dsm_engine *e, const dsm_change *changes, uint32_t k) {
   if ((rc = do_apply(e, changes, k))) return rc;
   HIPCHK(hipEventRecord(e->ev[1], e->stream));
   HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
+  // the dirty list (touched vertices, relabelled ones and their neighbours) is
+  // exactly the set whose label-sorted copies changed
+  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, e->dirty_list.p, &e->actl.p->ndirty, e->stream));
   HIPCHK(hipEventRecord(e->ev[2], e->stream));
   HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));

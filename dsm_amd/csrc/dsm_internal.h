@@ -11,6 +11,7 @@ constexpr int kWave = 64;       // CDNA wavefront
 constexpr int kMaxRel = 32;     // labels that can appear in an NLF requirement
 constexpr int kMaxVertexChanges = 1024;  // per vertex per apply round (host splits beyond)
 constexpr uint8_t kNoLabel = 255;  // device label of vertices whose label no query uses
+constexpr uint32_t kLabelSorted = 64;  // vertices of higher degree keep a label-sorted list copy
 
 // Matching plan of one (query, ordered query edge (u0,u1)) pair: positions 0
 // and 1 hold the seeded edge, later positions follow a greedy order in which
@@ -23,9 +24,10 @@ struct alignas(16) Plan {
   uint16_t sig[kMaxQ];   // candidate bitmap of each position
   uint16_t back[kMaxQ];  // earlier positions adjacent in the query
   uint16_t inj[kMaxQ];   // earlier positions with the same label (injectivity)
+  uint8_t lix[kMaxQ];    // dense label index of each position (label-sorted lists)
   uint32_t pad1[3];
 };
-static_assert(sizeof(Plan) == 128, "plan layout");
+static_assert(sizeof(Plan) == 144, "plan layout");
 
 // LDF/NLF requirement of one candidate bitmap: label, minimum degree and, for
 // each of up to 32 relevant labels (dense index), the minimum number of
@@ -89,6 +91,13 @@ struct DevGraph {
   uint32_t *deg;      // list length
   uint32_t *cap;      // slot capacity
   uint8_t *lab;       // vertex label (kNoLabel when no query uses it)
+  // Vertices of degree > kLabelSorted also keep their list sorted by (dense
+  // label, id) at lpool[start .. start + deg), with the bucket offsets of
+  // label index i at loff[v * (nbuckets + 1) + i] (bucket nbuckets - 1 holds
+  // labels no query uses).
+  uint32_t *lpool;
+  uint32_t *loff;
+  uint32_t nbuckets;
 };
 
 struct FilterTables {
@@ -175,6 +184,10 @@ hipError_t launch_apply_commit_failed(const DevGraph &g, const BatchDev &b, cons
 hipError_t launch_relabel(const DevGraph &g, const BatchDev &b, const FilterTables &ft,
                           const DirtyArgs &d, hipStream_t s);
 hipError_t launch_filter_full(const DevGraph &g, const FilterTables &ft, hipStream_t s);
+// Rebuilds the label-sorted copies of the listed vertices (list + count on the
+// device), or of every vertex when list is null.
+hipError_t launch_lbuild(const DevGraph &g, const uint8_t *lmap, const uint32_t *list, const uint32_t *count,
+                         hipStream_t s);
 hipError_t launch_filter_dirty(const DevGraph &g, const FilterTables &ft, const DirtyArgs &d,
                                hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);

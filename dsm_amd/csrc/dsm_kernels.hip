@@ -398,6 +398,69 @@ __global__ __launch_bounds__(256) void k_filter_heavy(DevGraph g, FilterTables f
   }
 }
 
+// ---------------------------------------------------- label-sorted copies
+
+// One wave per vertex of degree > kLabelSorted: counts its neighbours per
+// label bucket (LDS atomics), writes the bucket offsets, then scatters the
+// id-sorted list into lpool bucket by bucket, keeping id order inside each
+// bucket (per chunk, one ballot per distinct bucket present).
+__global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap, const uint32_t *list,
+                                                const uint32_t *count) {
+  __shared__ uint8_t s_lmap[256];
+  __shared__ uint32_t s_cnt[4][kMaxRel + 2];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = lmap[i];
+  __syncthreads();
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t nb = g.nbuckets;
+  const uint32_t total = list ? (*count < g.n ? *count : g.n) : g.n;
+  uint32_t *cnt = s_cnt[wv];
+  for (uint32_t t = blockIdx.x * 4 + wv; t < total; t += gridDim.x * 4) {
+    const uint32_t v = list ? list[t] : t;
+    const uint32_t d = g.deg[v];
+    if (d <= kLabelSorted) continue;
+    const uint32_t *A = g.pool + g.start[v];
+    uint32_t *out = g.lpool + g.start[v];
+    uint32_t *row = g.loff + (size_t)v * (nb + 1);
+    if (lane < nb) cnt[lane] = 0;
+    wave_sync();
+    for (uint32_t j = lane; j < d; j += 64) {
+      uint32_t b = s_lmap[g.lab[A[j]]];
+      if (b == 0xFF) b = nb - 1;
+      atomicAdd(&cnt[b], 1u);
+    }
+    wave_sync();
+    if (lane == 0) {
+      uint32_t run = 0;
+      for (uint32_t k = 0; k < nb; k++) {
+        const uint32_t c = cnt[k];
+        cnt[k] = run;
+        row[k] = run;
+        run += c;
+      }
+      row[nb] = run;
+    }
+    wave_sync();
+    for (uint32_t c0 = 0; c0 < d; c0 += 64) {
+      const uint32_t j = c0 + lane;
+      const bool valid = j < d;
+      const uint32_t x = valid ? A[j] : 0u;
+      uint32_t b = valid ? s_lmap[g.lab[x]] : 0u;
+      if (b == 0xFF) b = nb - 1;
+      uint64_t rest = ballot(valid);
+      while (rest) {
+        const uint32_t L = __shfl(b, (int)__builtin_ctzll(rest));
+        const uint64_t m = ballot(valid && b == L);
+        const uint32_t base = cnt[L];
+        if (valid && b == L) out[base + mbcnt(m)] = x;
+        wave_sync();
+        if (lane == 0) cnt[L] = base + (uint32_t)__popcll(m);
+        wave_sync();
+        rest &= ~m;
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------- seed
 
 // One thread per record (s,d): every plan whose seeded query edge (u0,u1) has
@@ -489,10 +552,12 @@ struct WaveHdr {
   uint32_t ci[kMaxQ];        // next chunk entry
   uint32_t cbase[kMaxQ];     // source index of the chunk's first lane
   uint64_t cmask[kMaxQ];     // lanes of the chunk that passed (k-th set bit = cand[k])
+  const uint32_t *src[kMaxQ]; // the list (or label bucket) the level scans
   uint16_t sig[kMaxQ];
   uint16_t back[kMaxQ];
   uint16_t inj[kMaxQ];
   uint8_t qv[kMaxQ];
+  uint8_t lix[kMaxQ];
   uint32_t n, query, plan, pad;
 };
 constexpr uint32_t kWin = 256;  // window entries staged in LDS per intersection
@@ -509,17 +574,42 @@ __host__ __device__ constexpr uint32_t wave_lds_bytes(uint32_t chunk_levels) {
   return (uint32_t)((sizeof(WaveHdr) + 15) / 16 * 16) + kWin * 4 + chunk_levels * 64 * 12;
 }
 
-__device__ __forceinline__ void init_level(WaveHdr &S, uint32_t l, uint32_t lo, uint32_t hi) {
+// Chooses the level's source: among the backward neighbours' lists, the one
+// with the fewest entries of the level's label (for a vertex of degree >
+// kLabelSorted, its label bucket in the label-sorted copy; otherwise the whole
+// list). Lane j examines backward position j; a wave minimum picks the
+// shortest, lowest position on ties.
+__device__ __forceinline__ void init_level(WaveHdr &S, const DevGraph &g, uint32_t l, uint32_t lo, uint32_t hi) {
+  const uint32_t lane = lane_id();
   const uint32_t bm = S.back[l];
-  uint32_t best = bm ? 0xFFFFFFFFu : 0u, bj = 0;  // no backward neighbour: empty scan
-  for (uint32_t j = 0; j < l; j++)
-    if ((bm >> j & 1) && S.fd[j] < best) {
-      best = S.fd[j];
-      bj = j;
+  uint64_t key = ~0ull;
+  uint32_t off = 0, sorted = 0;
+  if (lane < l && (bm >> lane & 1)) {
+    const uint32_t d = S.fd[lane];
+    uint32_t len = d;
+    off = S.fs[lane];
+    if (d > kLabelSorted) {
+      const uint32_t *row = g.loff + (size_t)S.f[lane] * (g.nbuckets + 1);
+      const uint32_t li = S.lix[l];
+      const uint32_t b0 = row[li], b1 = row[li + 1];
+      len = b1 - b0;
+      off += b0;
+      sorted = 1;
     }
+    key = ((uint64_t)len << 32) | lane;
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint64_t y = __shfl_xor(key, o);
+    key = y < key ? y : key;
+  }
+  const uint32_t bj = key == ~0ull ? 0u : (uint32_t)(key & 0xFFFFFFFFu);
+  const uint32_t best = key == ~0ull ? 0u : (uint32_t)(key >> 32);  // no backward neighbour: empty
+  const uint32_t boff = __shfl(off, (int)bj);
+  const uint32_t bsorted = __shfl(sorted, (int)bj);
   wave_sync();
-  if (lane_id() == 0) {
+  if (lane == 0) {
     S.srcj[l] = bj;
+    S.src[l] = (bsorted ? g.lpool : g.pool) + boff;
     S.pos[l] = lo;
     S.end[l] = hi < best ? hi : best;
     S.cn[l] = 0;
@@ -589,7 +679,7 @@ __device__ __forceinline__ bool test_chunk(const WaveMem &M, const DfsArgs &a, u
   const uint32_t i = p + lane;
   const uint32_t sj = S.srcj[l];
   bool ok = i < e;
-  w = ok ? a.g.pool[S.fs[sj] + i] : 0xFFFFFFFFu;
+  w = ok ? S.src[l][i] : 0xFFFFFFFFu;
   ok = ok && bit_of(a.bm, a.words, S.sig[l], w);
   uint32_t inj = S.inj[l];
   while (inj) {
@@ -837,6 +927,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
         S.back[lane] = P.back[lane];
         S.inj[lane] = P.inj[lane];
         S.qv[lane] = P.qv[lane];
+        S.lix[lane] = P.lix[lane];
       }
       if (lane == 0) {
         S.n = P.n;
@@ -858,7 +949,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
       if (lane == 0) lcount = 1;
       if (TUP) emit_tuples(S, a, n - 1, S.f[n - 1], lane == 0);
     } else {
-      init_level(S, L0, lo, hi);
+      init_level(S, a.g, L0, lo, hi);
       uint32_t l = L0;
       uint32_t tick = 0;
       for (;;) {
@@ -899,7 +990,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
           }
           wave_sync();
           l++;
-          init_level(S, l, 0, 0xFFFFFFFFu);
+          init_level(S, a.g, l, 0, 0xFFFFFFFFu);
           continue;
         }
         if (++tick >= kDonateEvery) {
@@ -1031,6 +1122,13 @@ hipError_t launch_filter_dirty(const DevGraph &g, const FilterTables &ft, const 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_filter_heavy, dim3(1024), dim3(256), 0, s, g, ft, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_lbuild(const DevGraph &g, const uint8_t *lmap, const uint32_t *list, const uint32_t *count,
+                         hipStream_t s) {
+  const uint32_t blocks = list ? 2048u : ((g.n + 3) / 4 < 8192 ? (g.n + 3) / 4 : 8192u);
+  hipLaunchKernelGGL(k_lbuild, dim3(blocks), dim3(256), 0, s, g, lmap, list, count);
   return hipGetLastError();
 }
 
