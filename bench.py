@@ -13,6 +13,11 @@ Legs (one JSON line on stdout, details on stderr):
   plain    the drop-in path as the reference runs it: one query per engine
            (one CDSM process per query), each engine doing the whole batch
            pipeline; run over the first --plain-steps timed batches
+  served   the same stream cut into requests of log-uniform size (1..16,384
+           records) arriving as a Poisson process at swept edge rates,
+           served by one dsm_server (continuous batching, admission by
+           estimated memory with 30% HBM headroom); per rate: achieved
+           updates/s, p50/p99 request latency (arrival to counts ready)
   cpu      the CPU oracle (single core) on one batch of the same stream
 """
 import argparse
@@ -121,6 +126,75 @@ def run_plain(w, warmup, steps, engine_kw):
     return {"lat": lat.tolist(), "records": recs, "counts": counts}
 
 
+def run_served(w, rates, first_batch, engine_kw, seed=7, max_requests=160, offered_s=6.0, hi=16384):
+    """Poisson arrivals of log-uniform-size requests at each rate in turn, on
+    one server; the graph keeps evolving from one rate to the next."""
+    import queue
+    import threading
+    from dsm_amd import engine as E
+    rng = np.random.default_rng(seed)
+    mean_size = (hi - 1) / np.log(hi)
+    eng = E.Engine((w.n, w.labels, w.eu, w.ev), qtuples(w), **engine_kw)
+    srv = E.Server(eng)
+    cur = {"b": first_batch, "buf": None, "pos": 0}
+
+    def take(k):
+        parts, need = [], k
+        while need > 0:
+            if cur["buf"] is None or cur["pos"] >= len(cur["buf"][1]):
+                cur["buf"] = w.batch(cur["b"] % w.num_batches)
+                cur["b"] += 1
+                cur["pos"] = 0
+            a, s_, d, l = cur["buf"]
+            j = min(len(s_), cur["pos"] + need)
+            parts.append((a[cur["pos"]:j], s_[cur["pos"]:j], d[cur["pos"]:j], l[cur["pos"]:j]))
+            need -= j - cur["pos"]
+            cur["pos"] = j
+        return tuple(np.concatenate([p[i] for p in parts]) for i in range(4))
+
+    out = []
+    try:
+        for rate in rates:
+            nreq = int(min(max_requests, max(20, rate * offered_s / mean_size)))
+            sizes = np.exp(rng.uniform(0, np.log(hi + 1), nreq)).astype(np.int64).clip(1, hi)
+            reqs = [take(int(k)) for k in sizes]
+            gaps = rng.exponential(mean_size / rate, nreq)
+            arrive = np.cumsum(gaps) - gaps[0]
+            ids = queue.Queue()
+
+            def submitter():
+                t0 = time.perf_counter()
+                for i, r in enumerate(reqs):
+                    dt = arrive[i] - (time.perf_counter() - t0)
+                    if dt > 0:
+                        time.sleep(dt)
+                    ids.put(srv.submit(*r))
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            infos = []
+            for _ in range(nreq):
+                st, pos, neg, info = srv.wait(ids.get())
+                if st != E.DSM_OK and st != E.DSM_EADMIT:
+                    raise RuntimeError("served request failed: %s" % srv.error())
+                infos.append(info)
+            th.join()
+            ok = [i for i in infos if i["status"] == E.DSM_OK]
+            lat = [i["t_done"] - i["t_submit"] for i in ok]
+            span = max(i["t_done"] for i in ok) - min(i["t_submit"] for i in infos)
+            recs = int(sum(i["num_changes"] for i in ok))
+            r = {"offered": rate, "requests": nreq, "rejected": nreq - len(ok), "records": recs,
+                 "value": round(recs / span, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
+                 "p99_ms": round(pct(lat, 99) * 1e3, 3),
+                 "mean_step_size": round(float(np.mean([i["step_size"] for i in ok])), 2)}
+            log("served rate %.0e: %s" % (rate, r))
+            out.append(r)
+    finally:
+        srv.close()
+        eng.close()
+    return out
+
+
 def run_cpu(w, batch_index, budget_s):
     """CPU oracle on one batch (all queries, until the budget is spent)."""
     from oracle import oracle as O
@@ -189,6 +263,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-rates", default="1e4,3e4,1e5,3e5,1e6,1e7")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--waves-per-block", type=int, default=4)
     ap.add_argument("--donate-min", type=int, default=0)
@@ -258,6 +334,16 @@ def main():
                         "p50_batch_ms": round(pct(pl["lat"], 50) * 1e3, 3),
                         "p99_batch_ms": round(pct(pl["lat"], 99) * 1e3, 3)}
         out["batched_vs_plain"] = round(value / plain_value, 3)
+
+    if not args.no_served:
+        rates = [float(x) for x in args.served_rates.split(",") if x]
+        sv = run_served(w, rates, args.warmup + args.steps, engine_kw)
+        out["served"] = sv
+        best = max(sv, key=lambda r: r["value"])
+        out["served_peak"] = {"value": best["value"], "offered": best["offered"], "p50_ms": best["p50_ms"],
+                              "p99_ms": best["p99_ms"]}
+        if "plain" in out:
+            out["served_vs_plain"] = round(best["value"] / out["plain"]["value"], 3)
 
     if not args.no_cpu:
         cpu = run_cpu(w, args.warmup, args.cpu_budget)

@@ -125,7 +125,7 @@ struct dsm_engine {
   PinnedBuf<DfsCtl> h_ctl;
   DevBuf<Task> queue;
   uint32_t qcap = 0, epoch = 0;
-  DevBuf<uint32_t> seed_plan, seed_s, seed_d;
+  DevBuf<uint32_t> seed_plan, seed_s, seed_d, seed_tag;
   uint32_t seed_cap = 0;
   DevBuf<unsigned long long> counts;
   PinnedBuf<unsigned long long> h_counts;
@@ -447,8 +447,8 @@ int upload_tables(dsm_engine *e) {
   HIPCHK(e->d_plan_key.alloc(e->plan_key.size()));
   HIPCHK(hipMemcpy(e->d_plan_key.p, e->plan_key.data(), e->plan_key.size() * 4, hipMemcpyHostToDevice));
   const uint32_t nq = (uint32_t)e->queries.size();
-  HIPCHK(e->counts.alloc((size_t)nq * 2));
-  HIPCHK(e->h_counts.ensure((size_t)nq * 2));
+  HIPCHK(e->counts.alloc((size_t)nq * 2 * kMaxSlots));
+  HIPCHK(e->h_counts.ensure((size_t)nq * 2 * kMaxSlots));
   HIPCHK(e->ctl.alloc(1));
   HIPCHK(e->h_ctl.ensure(1));
   e->qcap = e->opt.queue_capacity ? e->opt.queue_capacity : (1u << 20);
@@ -481,16 +481,22 @@ int ensure_seed_cap(dsm_engine *e, uint32_t nrec) {
     HIPCHK(e->seed_plan.alloc(c));
     HIPCHK(e->seed_s.alloc(c));
     HIPCHK(e->seed_d.alloc(c));
+    HIPCHK(e->seed_tag.alloc(c));
     e->seed_cap = c;
   }
   return DSM_OK;
 }
 
 // Seeds the records (device arrays rs/rd) and runs the persistent DFS.
-int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nrec, uint32_t sign,
-              hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
-  HIPCHK(hipEventRecord(ev0, e->stream));
+// A DFS round: reset the control block, seed one or more record lists (each
+// with its count tag, slot << 1 | sign), then one persistent DFS launch over
+// all their seeds. The seed table must hold the seeds of every list.
+int round_begin(dsm_engine *e) {
   HIPCHK(launch_ctl_reset(e->ctl.p, nullptr, nullptr, nullptr, 0, e->stream));
+  return DSM_OK;
+}
+
+int round_seed(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nrec, uint32_t tag) {
   SeedArgs sa;
   sa.g = e->dev_graph();
   sa.bm = e->bm.p;
@@ -504,10 +510,17 @@ int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nr
   sa.seed_plan = e->seed_plan.p;
   sa.seed_s = e->seed_s.p;
   sa.seed_d = e->seed_d.p;
+  sa.seed_tag = e->seed_tag.p;
   sa.seed_cap = e->seed_cap;
+  sa.tag = tag;
   sa.ctl = e->ctl.p;
   HIPCHK(launch_seed(sa, e->stream));
-  HIPCHK(hipEventRecord(ev1, e->stream));
+  return DSM_OK;
+}
+
+// nrec_hint sizes the grid: a round over a handful of records gets a grid of
+// one block per CU, so that few waves pay the start-up and termination atomics.
+int round_dfs(dsm_engine *e, uint32_t nrec_hint) {
   DfsArgs da;
   da.g = e->dev_graph();
   da.bm = e->bm.p;
@@ -516,20 +529,35 @@ int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nr
   da.seed_plan = e->seed_plan.p;
   da.seed_s = e->seed_s.p;
   da.seed_d = e->seed_d.p;
+  da.seed_tag = e->seed_tag.p;
   da.queue = e->queue.p;
   da.qcap = e->qcap;
   da.epoch = ++e->epoch;
   da.donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
-  da.sign = sign;
+  da.nq = (uint32_t)e->queries.size();
   da.watchdog = (unsigned long long)e->wall_khz * e->watchdog_ms;
   da.ctl = e->ctl.p;
   da.counts = e->counts.p;
   da.tuples = e->tuples;
   da.tuple_cap = e->tuple_cap;
   da.tuple_count = e->tuple_count.p;
-  HIPCHK(launch_dfs(da, e->dfs_blocks, e->dfs_wpb, e->max_qn, e->stream));
-  HIPCHK(hipEventRecord(ev2, e->stream));
+  const uint64_t want = ((uint64_t)nrec_hint * std::max<uint32_t>(e->max_plans_per_key, 1) * 8 + e->dfs_wpb - 1) /
+                        e->dfs_wpb;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(e->dfs_blocks, std::max<uint64_t>(want, (uint64_t)e->num_cu));
+  HIPCHK(launch_dfs(da, blocks, e->dfs_wpb, e->max_qn, e->stream));
   HIPCHK(hipMemcpyAsync(e->h_ctl.p, e->ctl.p, sizeof(DfsCtl), hipMemcpyDeviceToHost, e->stream));
+  return DSM_OK;
+}
+
+int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nrec, uint32_t tag,
+              hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
+  int rc;
+  HIPCHK(hipEventRecord(ev0, e->stream));
+  if ((rc = round_begin(e))) return rc;
+  if ((rc = round_seed(e, rs, rd, nrec, tag))) return rc;
+  HIPCHK(hipEventRecord(ev1, e->stream));
+  if ((rc = round_dfs(e, nrec))) return rc;
+  HIPCHK(hipEventRecord(ev2, e->stream));
   return DSM_OK;
 }
 
@@ -756,6 +784,21 @@ int upload_records(dsm_engine *e, const std::vector<uint32_t> &s, const std::vec
 }
 
 
+// One batch's update and candidate refresh, enqueued on the engine's stream:
+// reset the per-batch apply counters, apply (host delta + device merge/commit,
+// which waits for the commit's status), relabel, refilter the dirty vertices
+// and rebuild their label-sorted copies. No wait at the end.
+int apply_and_refresh(dsm_engine *e, const dsm_change *changes, uint32_t k) {
+  int rc;
+  HIPCHK(launch_ctl_reset(nullptr, e->actl.p, nullptr, nullptr, 0, e->stream));
+  if ((rc = do_apply(e, changes, k))) return rc;
+  HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
+  // the dirty list (touched vertices, relabelled ones and their neighbours) is
+  // exactly the set whose label-sorted copies changed
+  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, e->dirty_list.p, &e->actl.p->ndirty, e->stream));
+  return DSM_OK;
+}
+
 int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc *queries,
                 uint32_t num_queries) {
   int rc = 0;
@@ -868,7 +911,7 @@ int dsm_apply_batch(dsm_engine *e, const dsm_change *changes, uint32_t k) {
     }
   }
   const uint32_t nq = (uint32_t)e->queries.size();
-  HIPCHK(launch_ctl_reset(nullptr, e->actl.p, e->tuple_count.p, e->counts.p, nq * 2, e->stream));
+  HIPCHK(launch_ctl_reset(nullptr, nullptr, e->tuple_count.p, e->counts.p, nq * 2, e->stream));
   int rc;
   // 1. negatives in the pre-batch graph
   if (e->opt.count_negatives && !e->neg_s.empty()) {
@@ -885,20 +928,14 @@ int dsm_apply_batch(dsm_engine *e, const dsm_change *changes, uint32_t k) {
   }
   // 2-3. apply + relabel + filter
   HIPCHK(hipEventRecord(e->ev[0], e->stream));
-  if ((rc = do_apply(e, changes, k))) return rc;
-  HIPCHK(hipEventRecord(e->ev[1], e->stream));
-  HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
-  // the dirty list (touched vertices, relabelled ones and their neighbours) is
-  // exactly the set whose label-sorted copies changed
-  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, e->dirty_list.p, &e->actl.p->ndirty, e->stream));
+  if ((rc = apply_and_refresh(e, changes, k))) return rc;
   HIPCHK(hipEventRecord(e->ev[2], e->stream));
   HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
-  e->stats.ms_apply = ms;
-  HIPCHK(hipEventElapsedTime(&ms, e->ev[1], e->ev[2]));
-  e->stats.ms_filter = ms;
+  HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
+  e->stats.ms_apply = ms;  // update, relabel, refilter and label-sorted copies
+  e->stats.ms_filter = 0;
   e->stats.relabeled_vertices = e->h_actl.p->relabeled;
   e->stats.dirty_groups = e->h_actl.p->ndirty;
   e->stats.pool_entries_used = e->h_actl.p->pool_used;
@@ -1004,5 +1041,343 @@ int dsm_get_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words) {
 
 uint32_t dsm_num_vertices(const dsm_engine *e) { return e ? e->n : 0; }
 uint32_t dsm_num_queries(const dsm_engine *e) { return e ? (uint32_t)e->queries.size() : 0; }
+
+}  // extern "C"
+
+// ============================================================ served stream
+//
+// A request is one update batch. The server's worker thread forms steps out of
+// the pending requests, oldest first: a step admits a prefix of the queue whose
+// estimated intermediate memory fits the budget (dsm_admit_count) and that
+// fits the count slots. Within a step the requests keep their order and
+// semantics: request i's negatives are matched in the graph before its
+// update, its positives after it. Request i's positives and request i+1's
+// negatives see the same graph, so they share one DFS launch (seeds tagged
+// with their request's count slot); while that launch runs, the host prepares
+// the next request's update. Requests that arrive during a step wait for the
+// next one.
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct ServerReq {
+  uint64_t id = 0;
+  std::vector<dsm_change> ch;
+  uint64_t bytes = 0;
+  double t_submit = 0, t_start = 0, t_done = 0;
+  uint32_t step = 0, step_size = 0;
+  int status = 1;  // 1 while pending or running
+  std::vector<uint64_t> pos, neg;
+};
+
+}  // namespace
+
+struct dsm_server {
+  dsm_engine *e = nullptr;
+  uint64_t budget = 0;
+  uint32_t max_per_step = kMaxSlots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<ServerReq *> pending;
+  std::unordered_map<uint64_t, ServerReq *> reqs;
+  uint64_t next_id = 1;
+  uint32_t steps = 0;
+  bool stop = false;
+  std::thread worker;
+  // step resources
+  PinnedBuf<uint32_t> h_rec;
+  DevBuf<uint32_t> d_rec;
+  PinnedBuf<DfsCtl> h_ctls;
+  std::vector<hipEvent_t> ev_done;
+  hipEvent_t ev_start = nullptr;
+  std::string err;
+};
+
+namespace {
+
+uint64_t request_bytes(const dsm_engine *e, uint32_t k) {
+  // seed table (plan, src, dst, tag per seed) + records on the device + the
+  // host-side delta arrays uploaded for the update
+  return (uint64_t)k * ((uint64_t)std::max<uint32_t>(e->max_plans_per_key, 1) * 16 + 8 + 48);
+}
+
+// Serves one step. Returns DSM_OK or the first error (then every request of
+// the step carries it).
+int serve_step(dsm_server *sv, std::vector<ServerReq *> &G) {
+  dsm_engine *e = sv->e;
+  const uint32_t m = (uint32_t)G.size();
+  const uint32_t nq = (uint32_t)e->queries.size();
+  int rc;
+  HIPCHK(hipSetDevice(e->opt.device));
+  // records of every round, laid out once: round 0 = negatives of request 0;
+  // round i+1 = positives of request i followed by negatives of request i+1
+  std::vector<std::vector<uint32_t>> ps(m), pd(m), ns(m), nd(m);
+  for (uint32_t i = 0; i < m; i++)
+    for (const dsm_change &c : G[i]->ch) {
+      if (c.add) {
+        ps[i].push_back(c.src);
+        pd[i].push_back(c.dst);
+      } else if (e->opt.count_negatives) {
+        ns[i].push_back(c.src);
+        nd[i].push_back(c.dst);
+      }
+    }
+  struct Part {
+    size_t off;
+    uint32_t n, tag;
+  };
+  std::vector<std::vector<Part>> rounds(m + 1);
+  size_t total = 0;
+  uint64_t max_round = 0;
+  auto add_part = [&](uint32_t r, const std::vector<uint32_t> &s, const std::vector<uint32_t> &d, uint32_t tag) {
+    if (s.empty()) return;
+    rounds[r].push_back({total, (uint32_t)s.size(), tag});
+    total += 2 * s.size();
+  };
+  add_part(0, ns[0], nd[0], 0u);
+  for (uint32_t i = 0; i < m; i++) {
+    add_part(i + 1, ps[i], pd[i], (i << 1) | 1u);
+    if (i + 1 < m) add_part(i + 1, ns[i + 1], nd[i + 1], (i + 1) << 1);
+  }
+  for (auto &r : rounds) {
+    uint64_t n = 0;
+    for (auto &p : r) n += p.n;
+    max_round = std::max(max_round, n);
+  }
+  if ((rc = ensure_seed_cap(e, (uint32_t)std::min<uint64_t>(max_round, 0xFFFFFFFFull)))) return rc;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(sv->h_rec.ensure(total + 2));
+  HIPCHK(sv->d_rec.ensure(total + 2));
+  {
+    size_t o = 0;
+    auto put = [&](const std::vector<uint32_t> &s, const std::vector<uint32_t> &d) {
+      if (s.empty()) return;
+      std::copy(s.begin(), s.end(), sv->h_rec.p + o);
+      std::copy(d.begin(), d.end(), sv->h_rec.p + o + s.size());
+      o += 2 * s.size();
+    };
+    put(ns[0], nd[0]);
+    for (uint32_t i = 0; i < m; i++) {
+      put(ps[i], pd[i]);
+      if (i + 1 < m) put(ns[i + 1], nd[i + 1]);
+    }
+  }
+  HIPCHK(hipMemcpyAsync(sv->d_rec.p, sv->h_rec.p, (total + 2) * 4, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(sv->h_ctls.ensure(m + 1));
+  while (sv->ev_done.size() < m + 1) {
+    hipEvent_t ev;
+    HIPCHK(hipEventCreate(&ev));
+    sv->ev_done.push_back(ev);
+  }
+  if (!sv->ev_start) HIPCHK(hipEventCreate(&sv->ev_start));
+  HIPCHK(launch_ctl_reset(nullptr, nullptr, nullptr, e->counts.p, m * nq * 2, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  const double t0 = now_s();
+  HIPCHK(hipEventRecord(sv->ev_start, e->stream));
+  auto run = [&](uint32_t r) -> int {
+    int rc2;
+    if (rounds[r].empty()) {
+      HIPCHK(hipEventRecord(sv->ev_done[r], e->stream));
+      return DSM_OK;
+    }
+    if ((rc2 = round_begin(e))) return rc2;
+    uint32_t n = 0;
+    for (auto &p : rounds[r]) {
+      if ((rc2 = round_seed(e, sv->d_rec.p + p.off, sv->d_rec.p + p.off + p.n, p.n, p.tag))) return rc2;
+      n += p.n;
+    }
+    if ((rc2 = round_dfs(e, n))) return rc2;
+    HIPCHK(hipMemcpyAsync(sv->h_ctls.p + r, e->ctl.p, sizeof(DfsCtl), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipEventRecord(sv->ev_done[r], e->stream));
+    return DSM_OK;
+  };
+  if ((rc = run(0))) return rc;
+  for (uint32_t i = 0; i < m; i++) {
+    if ((rc = apply_and_refresh(e, G[i]->ch.data(), (uint32_t)G[i]->ch.size()))) return rc;
+    if ((rc = run(i + 1))) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(e->h_counts.p, e->counts.p, (size_t)m * nq * 2 * 8, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  for (uint32_t r = 0; r <= m; r++) {
+    if (rounds[r].empty()) continue;
+    const DfsCtl &c = sv->h_ctls.p[r];
+    if (c.seed_overflow.v) return e->fail_msg(DSM_ELIMIT, "seed table overflow in a served step");
+    if (c.abort.v) return e->fail_msg(DSM_EDEVICE, "DFS watchdog expired in a served step");
+  }
+  for (uint32_t i = 0; i < m; i++) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, sv->ev_start, sv->ev_done[i + 1]));
+    G[i]->t_done = t0 + ms * 1e-3;
+    G[i]->pos.assign(nq, 0);
+    G[i]->neg.assign(nq, 0);
+    for (uint32_t q = 0; q < nq; q++) {
+      G[i]->pos[q] = e->h_counts.p[((size_t)i * nq + q) * 2 + 1];
+      G[i]->neg[q] = e->h_counts.p[((size_t)i * nq + q) * 2 + 0];
+    }
+  }
+  return DSM_OK;
+}
+
+void server_loop(dsm_server *sv) {
+  for (;;) {
+    std::vector<ServerReq *> G;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
+      if (sv->pending.empty()) return;  // stop requested and drained
+      std::vector<uint64_t> bytes;
+      for (size_t i = 0; i < sv->pending.size() && i < sv->max_per_step; i++) bytes.push_back(sv->pending[i]->bytes);
+      const uint32_t n = dsm_admit_count(bytes.data(), (uint32_t)bytes.size(), sv->budget, sv->max_per_step);
+      if (n == 0) {
+        // too large even alone: rejected without touching the graph
+        ServerReq *r = sv->pending.front();
+        sv->pending.pop_front();
+        r->status = DSM_EADMIT;
+        r->t_start = r->t_done = now_s();
+        sv->cv_done.notify_all();
+        continue;
+      }
+      for (uint32_t i = 0; i < n; i++) {
+        G.push_back(sv->pending.front());
+        sv->pending.pop_front();
+      }
+    }
+    const double ts = now_s();
+    const uint32_t step = sv->steps++;
+    for (auto *r : G) {
+      r->t_start = ts;
+      r->step = step;
+      r->step_size = (uint32_t)G.size();
+    }
+    const int rc = serve_step(sv, G);
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if (rc != DSM_OK) sv->err = sv->e->err;
+      for (auto *r : G) {
+        r->status = rc;
+        if (rc != DSM_OK) r->t_done = now_s();
+      }
+    }
+    sv->cv_done.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t dsm_admit_count(const uint64_t *request_bytes, uint32_t num_pending, uint64_t budget,
+                         uint32_t max_per_step) {
+  uint64_t used = 0;
+  uint32_t n = 0;
+  if (max_per_step == 0 || max_per_step > kMaxSlots) max_per_step = kMaxSlots;
+  while (n < num_pending && n < max_per_step) {
+    if (request_bytes[n] > budget - used) break;
+    used += request_bytes[n];
+    n++;
+  }
+  return n;
+}
+
+uint64_t dsm_estimate_request_bytes(const dsm_engine *e, uint32_t num_changes) {
+  return e ? request_bytes(e, num_changes) : 0;
+}
+
+int dsm_server_create(dsm_engine *e, const dsm_server_options *opt, dsm_server **out) {
+  if (!e || !out) return DSM_EINVAL;
+  *out = nullptr;
+  if (e->tuple_cap) return e->fail_msg(DSM_EINVAL, "served mode keeps counts only (tuple_capacity must be 0)");
+  HIPCHK(hipSetDevice(e->opt.device));
+  std::unique_ptr<dsm_server> sv(new dsm_server());
+  sv->e = e;
+  size_t freeb = 0, totalb = 0;
+  HIPCHK(hipMemGetInfo(&freeb, &totalb));
+  const double headroom = (opt && opt->hbm_headroom > 0) ? opt->hbm_headroom : 0.30;
+  const double reserve = headroom * (double)totalb;
+  if (opt && opt->budget_bytes) sv->budget = opt->budget_bytes;
+  else sv->budget = (double)freeb > reserve ? (uint64_t)((double)freeb - reserve) : 0;
+  if (opt && opt->max_requests_per_step) sv->max_per_step = std::min<uint32_t>(opt->max_requests_per_step, kMaxSlots);
+  dsm_server *p = sv.get();
+  p->worker = std::thread(server_loop, p);
+  *out = sv.release();
+  return DSM_OK;
+}
+
+int dsm_server_submit(dsm_server *sv, const dsm_change *changes, uint32_t num_changes, uint64_t *id) {
+  if (!sv || (!changes && num_changes) || !id) return DSM_EINVAL;
+  const uint32_t n = sv->e->n;
+  for (uint32_t i = 0; i < num_changes; i++)
+    if (changes[i].src >= n || changes[i].dst >= n) return DSM_EINVAL;
+  ServerReq *r = new ServerReq();
+  r->ch.assign(changes, changes + num_changes);
+  r->bytes = request_bytes(sv->e, num_changes);
+  r->t_submit = now_s();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    r->id = sv->next_id++;
+    sv->reqs[r->id] = r;
+    sv->pending.push_back(r);
+    *id = r->id;
+  }
+  sv->cv_work.notify_one();
+  return DSM_OK;
+}
+
+int dsm_server_wait(dsm_server *sv, uint64_t id, dsm_request_info *info, uint64_t *pos, uint64_t *neg) {
+  if (!sv) return DSM_EINVAL;
+  ServerReq *r;
+  {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return DSM_EINVAL;
+    r = it->second;
+    sv->cv_done.wait(lk, [&] { return r->status != 1; });
+    sv->reqs.erase(it);
+  }
+  const uint32_t nq = (uint32_t)sv->e->queries.size();
+  if (info) {
+    info->id = r->id;
+    info->status = r->status;
+    info->t_submit = r->t_submit;
+    info->t_start = r->t_start;
+    info->t_done = r->t_done;
+    info->step = r->step;
+    info->step_size = r->step_size;
+    info->num_changes = (uint32_t)r->ch.size();
+  }
+  for (uint32_t q = 0; q < nq; q++) {
+    if (pos) pos[q] = r->status == DSM_OK ? r->pos[q] : 0;
+    if (neg) neg[q] = r->status == DSM_OK ? r->neg[q] : 0;
+  }
+  const int st = r->status;
+  delete r;
+  return st;
+}
+
+const char *dsm_server_error(const dsm_server *sv) { return sv ? sv->err.c_str() : ""; }
+
+void dsm_server_destroy(dsm_server *sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  for (auto &kv : sv->reqs) delete kv.second;
+  for (auto ev : sv->ev_done) (void)hipEventDestroy(ev);
+  if (sv->ev_start) (void)hipEventDestroy(sv->ev_start);
+  delete sv;
+}
 
 }  // extern "C"

@@ -46,11 +46,13 @@ struct alignas(128) Task {
   uint32_t level;
   uint32_t lo, hi;
   uint32_t f[kMaxQ];
+  uint32_t tag;          // count slot << 1 | sign (1 positive, 0 negative)
   uint32_t ready;
-  uint32_t pad[11];
+  uint32_t pad[10];
 };
 static_assert(sizeof(Task) == 128, "task slot layout");
-constexpr int kTaskWords = 4 + kMaxQ;  // payload words (20)
+constexpr int kTaskWords = 5 + kMaxQ;  // payload words (21)
+constexpr uint32_t kMaxSlots = 64;     // count slots: requests served in one step
 
 // Device-resident control block of one seed + DFS round. Every counter that
 // waves update or poll concurrently has a 128-byte line of its own, so polling
@@ -138,10 +140,11 @@ struct DfsArgs {
   uint32_t qcap;
   uint32_t epoch;
   uint32_t donate_min;
-  uint32_t sign;                 // 1 positive, 0 negative
+  uint32_t nq;                   // queries (count layout)
+  const uint32_t *seed_tag;      // per seed: count slot << 1 | sign
   unsigned long long watchdog;   // wall-clock ticks an idle wave may wait
   DfsCtl *ctl;
-  unsigned long long *counts;    // [nq][2]: index q*2+sign
+  unsigned long long *counts;    // [slot][nq][2]: index (slot*nq + q)*2 + sign
   uint32_t *tuples;              // [tuple_cap][kMaxQ+2] or null
   unsigned long long tuple_cap;
   unsigned long long *tuple_count;  // tuples found this batch
@@ -160,7 +163,9 @@ struct SeedArgs {
   uint32_t *seed_plan;
   uint32_t *seed_s;
   uint32_t *seed_d;
+  uint32_t *seed_tag;
   uint32_t seed_cap;
+  uint32_t tag;               // written to every seed of these records
   DfsCtl *ctl;
 };
 

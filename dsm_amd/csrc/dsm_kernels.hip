@@ -498,6 +498,7 @@ __global__ __launch_bounds__(256) void k_seed(SeedArgs a) {
         a.seed_plan[o] = p;
         a.seed_s[o] = s;
         a.seed_d[o] = d;
+        a.seed_tag[o] = a.tag;
         o++;
       }
     }
@@ -558,7 +559,7 @@ struct WaveHdr {
   uint16_t inj[kMaxQ];
   uint8_t qv[kMaxQ];
   uint8_t lix[kMaxQ];
-  uint32_t n, query, plan, pad;
+  uint32_t n, query, plan, tag;
 };
 constexpr uint32_t kWin = 256;  // window entries staged in LDS per intersection
 
@@ -725,7 +726,7 @@ __device__ void emit_tuples(const WaveHdr &S, const DfsArgs &a, uint32_t l, uint
   if (idx >= a.tuple_cap) return;
   uint32_t *t = a.tuples + idx * (kMaxQ + 2);
   t[0] = S.query;
-  t[1] = a.sign;
+  t[1] = S.tag;
   for (uint32_t k = S.n; k < (uint32_t)kMaxQ; k++) t[2 + k] = 0xFFFFFFFFu;
   for (uint32_t j = 0; j < l; j++) t[2 + S.qv[j]] = S.f[j];
   t[2 + S.qv[l]] = w;
@@ -804,7 +805,8 @@ __device__ void try_donate(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l
   else if (lane == 1) word = lv;
   else if (lane == 2) word = mid;
   else if (lane == 3) word = e;
-  else if (lane < (uint32_t)kTaskWords) word = S.f[lane - 4];
+  else if (lane < 4u + kMaxQ) word = S.f[lane - 4];
+  else if (lane == 4u + kMaxQ) word = S.tag;
   if (lane < (uint32_t)kTaskWords) reinterpret_cast<uint32_t *>(&a.queue[slot])[lane] = word;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   if (lane == 0) {
@@ -838,6 +840,10 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
   // reports the error (the table is sized so that it cannot happen)
   if (a.ctl->seed_overflow.v) return;
   const uint32_t nseeds = a.ctl->nseeds.v;
+  // Seeds are claimed from a counter, never assigned by wave index: the grid
+  // is sized to what the occupancy query says fits, but nothing guarantees
+  // that every block is resident at once, and a seed reserved for a block
+  // that has not started would keep the resident waves waiting for it.
   bool seeds_left = true;
   // Donated tasks are handed out by ticket: a wave out of work takes the next
   // ticket with one atomic add and then waits on the ready flag inside that
@@ -903,7 +909,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
       continue;
     }
     // ---- load the task
-    uint32_t plan, L0, lo, hi, fw = 0;
+    uint32_t plan, L0, lo, hi, tag, fw = 0;
     if (got == 1) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       const uint32_t word =
@@ -912,14 +918,17 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
       L0 = __builtin_amdgcn_readlane(word, 1);
       lo = __builtin_amdgcn_readlane(word, 2);
       hi = __builtin_amdgcn_readlane(word, 3);
+      tag = __builtin_amdgcn_readlane(word, 4 + kMaxQ);
       fw = __shfl(word, (int)((lane + 4) & 63));
     } else {
       plan = a.seed_plan[slot];
+      tag = a.seed_tag[slot];
       L0 = 2;
       lo = 0;
       hi = 0xFFFFFFFFu;
       fw = lane == 0 ? a.seed_s[slot] : a.seed_d[slot];
     }
+    if (lane == 0) S.tag = tag;
     if (bcast(S.plan) != plan) {
       const Plan &P = a.plans[plan];
       if (lane < (uint32_t)kMaxQ) {
@@ -1039,7 +1048,9 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
       ld += __shfl_xor(ld, off);
     }
     if (lane == 0) {
-      if (tot) atomicAdd(&a.counts[(size_t)bcast(S.query) * 2 + a.sign], (unsigned long long)tot);
+      const uint32_t tg = S.tag;
+      if (tot)
+        atomicAdd(&a.counts[((size_t)(tg >> 1) * a.nq + S.query) * 2 + (tg & 1)], (unsigned long long)tot);
       atomicAdd(&a.ctl->scanned.v, (unsigned long long)sc);
       atomicAdd(&a.ctl->loads.v, (unsigned long long)ld);
       atomicSub(&a.ctl->work_left.v, 1);

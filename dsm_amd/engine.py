@@ -36,6 +36,8 @@ EXPORTS = (
     "dsm_match_batch", "dsm_process_batch", "dsm_get_tuples", "dsm_get_stats", "dsm_get_graph",
     "dsm_get_candidates", "dsm_num_vertices", "dsm_num_queries", "dsm_file_read", "dsm_file_free",
     "dsm_file_graph", "dsm_file_query", "dsm_file_num_batches", "dsm_file_batch", "dsm_file_error",
+    "dsm_server_create", "dsm_server_submit", "dsm_server_wait", "dsm_server_error", "dsm_server_destroy",
+    "dsm_admit_count", "dsm_estimate_request_bytes",
 )
 
 
@@ -74,6 +76,20 @@ class Stats(C.Structure):
                 ("ms_seed", C.c_float), ("ms_dfs_pos", C.c_float), ("ms_dfs_neg", C.c_float),
                 ("pool_entries_used", C.c_uint64), ("pool_entries_capacity", C.c_uint64),
                 ("dfs_entries", C.c_uint64), ("dfs_search_loads", C.c_uint64), ("dfs_launches", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("hbm_headroom", C.c_double),
+                ("max_requests_per_step", C.c_uint32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double), ("step", C.c_uint32), ("step_size", C.c_uint32),
+                ("num_changes", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -131,6 +147,19 @@ def lib():
     L.dsm_file_batch.argtypes = [vp, C.c_uint32, C.POINTER(C.POINTER(Change)), C.POINTER(C.c_uint32)]
     L.dsm_file_error.restype = C.c_char_p
     L.dsm_file_error.argtypes = []
+    L.dsm_server_create.restype = C.c_int
+    L.dsm_server_create.argtypes = [vp, C.POINTER(ServerOptions), C.POINTER(vp)]
+    L.dsm_server_submit.restype = C.c_int
+    L.dsm_server_submit.argtypes = [vp, C.POINTER(Change), C.c_uint32, C.POINTER(C.c_uint64)]
+    L.dsm_server_wait.restype = C.c_int
+    L.dsm_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(RequestInfo), vp, vp]
+    L.dsm_server_error.restype = C.c_char_p
+    L.dsm_server_error.argtypes = [vp]
+    L.dsm_server_destroy.argtypes = [vp]
+    L.dsm_admit_count.restype = C.c_uint32
+    L.dsm_admit_count.argtypes = [vp, C.c_uint32, C.c_uint64, C.c_uint32]
+    L.dsm_estimate_request_bytes.restype = C.c_uint64
+    L.dsm_estimate_request_bytes.argtypes = [vp, C.c_uint32]
     _LIB = L
     return L
 
@@ -305,3 +334,55 @@ class Engine:
         self._check(lib().dsm_get_candidates(self._h, q, u, words.ctypes.data))
         bits = np.unpackbits(words.view(np.uint8), bitorder="little")
         return bits[: self.n].astype(bool)
+
+
+def admit_count(request_bytes, budget, max_per_step=0):
+    """The server's admission rule (dsm_admit_count) on a list of request sizes."""
+    b = np.ascontiguousarray(np.asarray(request_bytes, np.uint64))
+    return int(lib().dsm_admit_count(b.ctypes.data, len(b), int(budget), int(max_per_step)))
+
+
+class Server:
+    """Served stream over one Engine (dsm_server_*): submit update batches from
+    any thread, wait for each one's counts. The engine is not used directly
+    while the server exists."""
+
+    def __init__(self, engine, budget_bytes=0, hbm_headroom=0.0, max_requests_per_step=0):
+        o = ServerOptions(int(budget_bytes), float(hbm_headroom), int(max_requests_per_step))
+        h = C.c_void_p()
+        rc = lib().dsm_server_create(engine._h, C.byref(o), C.byref(h))
+        if rc != DSM_OK:
+            raise DsmError(rc, lib().dsm_last_error(engine._h).decode())
+        self._h = h
+        self.engine = engine
+        self.nq = engine.nq
+
+    def estimate_bytes(self, num_changes):
+        return int(lib().dsm_estimate_request_bytes(self.engine._h, int(num_changes)))
+
+    def submit(self, add, src, dst, label):
+        rec = changes_array(add, src, dst, label)
+        rid = C.c_uint64()
+        rc = lib().dsm_server_submit(self._h, rec.ctypes.data_as(C.POINTER(Change)), rec.shape[0], C.byref(rid))
+        if rc != DSM_OK:
+            raise DsmError(rc, "submit rejected")
+        return rid.value
+
+    def wait(self, rid):
+        """(status, pos u64[nq], neg u64[nq], info dict) of request rid."""
+        pos = np.zeros(self.nq, np.uint64)
+        neg = np.zeros(self.nq, np.uint64)
+        info = RequestInfo()
+        st = lib().dsm_server_wait(self._h, int(rid), C.byref(info), pos.ctypes.data, neg.ctypes.data)
+        return st, pos, neg, info.as_dict()
+
+    def error(self):
+        return lib().dsm_server_error(self._h).decode()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dsm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
