@@ -186,7 +186,8 @@ def run_served(w, rates, first_batch, engine_kw, seed=7, max_requests=160, offer
             r = {"offered": rate, "requests": nreq, "rejected": nreq - len(ok), "records": recs,
                  "value": round(recs / span, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
                  "p99_ms": round(pct(lat, 99) * 1e3, 3),
-                 "mean_step_size": round(float(np.mean([i["step_size"] for i in ok])), 2)}
+                 "fused_frac": round(float(np.mean([i["fused"] for i in ok])), 3),
+                 "mean_queue_depth": round(float(np.mean([i["queue_depth"] for i in ok])), 2)}
             log("served rate %.0e: %s" % (rate, r))
             out.append(r)
     finally:
@@ -264,7 +265,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--served-rates", default="1e4,3e4,1e5,3e5,1e6,1e7")
+    ap.add_argument("--served-rates", default="1e4,3e4,5e4,7e4,1e5,3e5,1e6,1e7")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--waves-per-block", type=int, default=4)
     ap.add_argument("--donate-min", type=int, default=0)
@@ -344,6 +345,13 @@ def main():
                               "p99_ms": best["p99_ms"]}
         if "plain" in out:
             out["served_vs_plain"] = round(best["value"] / out["plain"]["value"], 3)
+            # the highest served throughput whose p99 request latency is at most
+            # the plain path's p99 batch latency
+            fit = [r for r in sv if r["p99_ms"] <= out["plain"]["p99_batch_ms"]]
+            if fit:
+                b2 = max(fit, key=lambda r: r["value"])
+                out["served_at_plain_p99"] = {"value": b2["value"], "offered": b2["offered"], "p99_ms": b2["p99_ms"],
+                                              "vs_plain": round(b2["value"] / out["plain"]["value"], 3)}
 
     if not args.no_cpu:
         cpu = run_cpu(w, args.warmup, args.cpu_budget)

@@ -725,9 +725,7 @@ int grow_pool(dsm_engine *e, unsigned long long extra) {
   return DSM_OK;
 }
 
-int do_apply(dsm_engine *e, const dsm_change *ch, uint32_t k) {
-  HostDelta d;
-  host_delta(e, ch, k, d);
+int do_apply_delta(dsm_engine *e, const HostDelta &d) {
   const uint32_t T = (uint32_t)d.tv.size();
   // split into rounds of at most kMaxVertexChanges inserts and deletes per vertex
   uint32_t rounds = 1;
@@ -784,19 +782,25 @@ int upload_records(dsm_engine *e, const std::vector<uint32_t> &s, const std::vec
 }
 
 
-// One batch's update and candidate refresh, enqueued on the engine's stream:
-// reset the per-batch apply counters, apply (host delta + device merge/commit,
-// which waits for the commit's status), relabel, refilter the dirty vertices
-// and rebuild their label-sorted copies. No wait at the end.
-int apply_and_refresh(dsm_engine *e, const dsm_change *changes, uint32_t k) {
+// One batch's update and candidate refresh from its host delta, enqueued on
+// the engine's stream: reset the per-batch apply counters, apply (device
+// merge/commit, which waits for the commit's status), relabel, refilter the
+// dirty vertices and rebuild their label-sorted copies. No wait at the end.
+int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d) {
   int rc;
   HIPCHK(launch_ctl_reset(nullptr, e->actl.p, nullptr, nullptr, 0, e->stream));
-  if ((rc = do_apply(e, changes, k))) return rc;
+  if ((rc = do_apply_delta(e, d))) return rc;
   HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
   // the dirty list (touched vertices, relabelled ones and their neighbours) is
   // exactly the set whose label-sorted copies changed
   HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, e->dirty_list.p, &e->actl.p->ndirty, e->stream));
   return DSM_OK;
+}
+
+int apply_and_refresh(dsm_engine *e, const dsm_change *changes, uint32_t k) {
+  HostDelta d;
+  host_delta(e, changes, k, d);
+  return apply_delta_and_refresh(e, d);
 }
 
 int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc *queries,
@@ -1046,16 +1050,16 @@ uint32_t dsm_num_queries(const dsm_engine *e) { return e ? (uint32_t)e->queries.
 
 // ============================================================ served stream
 //
-// A request is one update batch. The server's worker thread forms steps out of
-// the pending requests, oldest first: a step admits a prefix of the queue whose
-// estimated intermediate memory fits the budget (dsm_admit_count) and that
-// fits the count slots. Within a step the requests keep their order and
-// semantics: request i's negatives are matched in the graph before its
-// update, its positives after it. Request i's positives and request i+1's
-// negatives see the same graph, so they share one DFS launch (seeds tagged
-// with their request's count slot); while that launch runs, the host prepares
-// the next request's update. Requests that arrive during a step wait for the
-// next one.
+// A request is one update batch. The worker thread keeps a rolling pipeline
+// over the submitted requests, in submission order, with the semantics of
+// dsm_process_batch for each: request R's negatives are matched in the graph
+// before R's update, its positives after it. The positives of the request in
+// flight (P, already applied) and the negatives of the next one (R) see the
+// same graph, so one DFS launch matches both, each seed tagged with its
+// request's count slot. While that launch runs, the host computes R's delta;
+// when it ends, P's counts are complete and are published at once, and R's
+// update is applied. A request is admitted when it and the one in flight fit
+// the memory budget together; one that does not fit even alone is rejected.
 
 #include <chrono>
 #include <condition_variable>
@@ -1073,10 +1077,11 @@ double now_s() {
 struct ServerReq {
   uint64_t id = 0;
   std::vector<dsm_change> ch;
+  std::vector<uint32_t> ps, pd, ns, nd;  // positive and negative records
   uint64_t bytes = 0;
   double t_submit = 0, t_start = 0, t_done = 0;
-  uint32_t step = 0, step_size = 0;
-  int status = 1;  // 1 while pending or running
+  uint32_t queue_depth = 0, fused = 0, slot = 0;
+  int status = 1;  // 1 while pending or in flight
   std::vector<uint64_t> pos, neg;
 };
 
@@ -1085,21 +1090,16 @@ struct ServerReq {
 struct dsm_server {
   dsm_engine *e = nullptr;
   uint64_t budget = 0;
-  uint32_t max_per_step = kMaxSlots;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<ServerReq *> pending;
   std::unordered_map<uint64_t, ServerReq *> reqs;
   uint64_t next_id = 1;
-  uint32_t steps = 0;
   bool stop = false;
+  int failed = DSM_OK;  // after a device error every later request fails with it
   std::thread worker;
-  // step resources
   PinnedBuf<uint32_t> h_rec;
   DevBuf<uint32_t> d_rec;
-  PinnedBuf<DfsCtl> h_ctls;
-  std::vector<hipEvent_t> ev_done;
-  hipEvent_t ev_start = nullptr;
   std::string err;
 };
 
@@ -1111,164 +1111,124 @@ uint64_t request_bytes(const dsm_engine *e, uint32_t k) {
   return (uint64_t)k * ((uint64_t)std::max<uint32_t>(e->max_plans_per_key, 1) * 16 + 8 + 48);
 }
 
-// Serves one step. Returns DSM_OK or the first error (then every request of
-// the step carries it).
-int serve_step(dsm_server *sv, std::vector<ServerReq *> &G) {
+void publish(dsm_server *sv, ServerReq *r, int status) {
+  std::lock_guard<std::mutex> lk(sv->mu);
+  r->status = status;
+  r->t_done = now_s();
+  sv->cv_done.notify_all();
+}
+
+// One DFS round over the current graph: the positives of P (if any) and the
+// negatives of R (if any); then P's count slot is copied to the host. Waits
+// for the round; while it runs, computes R's delta into *d.
+int serve_round(dsm_server *sv, ServerReq *P, ServerReq *R, HostDelta *d) {
   dsm_engine *e = sv->e;
-  const uint32_t m = (uint32_t)G.size();
   const uint32_t nq = (uint32_t)e->queries.size();
   int rc;
-  HIPCHK(hipSetDevice(e->opt.device));
-  // records of every round, laid out once: round 0 = negatives of request 0;
-  // round i+1 = positives of request i followed by negatives of request i+1
-  std::vector<std::vector<uint32_t>> ps(m), pd(m), ns(m), nd(m);
-  for (uint32_t i = 0; i < m; i++)
-    for (const dsm_change &c : G[i]->ch) {
-      if (c.add) {
-        ps[i].push_back(c.src);
-        pd[i].push_back(c.dst);
-      } else if (e->opt.count_negatives) {
-        ns[i].push_back(c.src);
-        nd[i].push_back(c.dst);
-      }
+  const uint32_t np = P ? (uint32_t)P->ps.size() : 0u, nn = R ? (uint32_t)R->ns.size() : 0u;
+  if (R) HIPCHK(launch_ctl_reset(nullptr, nullptr, nullptr, e->counts.p + (size_t)R->slot * nq * 2, nq * 2,
+                                 e->stream));
+  if (np + nn) {
+    if ((rc = ensure_seed_cap(e, np + nn))) return rc;
+    HIPCHK(sv->h_rec.ensure(2 * (size_t)(np + nn) + 2));
+    HIPCHK(sv->d_rec.ensure(2 * (size_t)(np + nn) + 2));
+    uint32_t *h = sv->h_rec.p;
+    if (np) {
+      std::copy(P->ps.begin(), P->ps.end(), h);
+      std::copy(P->pd.begin(), P->pd.end(), h + np);
     }
-  struct Part {
-    size_t off;
-    uint32_t n, tag;
-  };
-  std::vector<std::vector<Part>> rounds(m + 1);
-  size_t total = 0;
-  uint64_t max_round = 0;
-  auto add_part = [&](uint32_t r, const std::vector<uint32_t> &s, const std::vector<uint32_t> &d, uint32_t tag) {
-    if (s.empty()) return;
-    rounds[r].push_back({total, (uint32_t)s.size(), tag});
-    total += 2 * s.size();
-  };
-  add_part(0, ns[0], nd[0], 0u);
-  for (uint32_t i = 0; i < m; i++) {
-    add_part(i + 1, ps[i], pd[i], (i << 1) | 1u);
-    if (i + 1 < m) add_part(i + 1, ns[i + 1], nd[i + 1], (i + 1) << 1);
+    if (nn) {
+      std::copy(R->ns.begin(), R->ns.end(), h + 2 * np);
+      std::copy(R->nd.begin(), R->nd.end(), h + 2 * np + nn);
+    }
+    HIPCHK(hipMemcpyAsync(sv->d_rec.p, h, 2 * (size_t)(np + nn) * 4, hipMemcpyHostToDevice, e->stream));
+    if ((rc = round_begin(e))) return rc;
+    if (np && (rc = round_seed(e, sv->d_rec.p, sv->d_rec.p + np, np, (P->slot << 1) | 1u))) return rc;
+    if (nn && (rc = round_seed(e, sv->d_rec.p + 2 * np, sv->d_rec.p + 2 * np + nn, nn, R->slot << 1))) return rc;
+    if ((rc = round_dfs(e, np + nn))) return rc;
   }
-  for (auto &r : rounds) {
-    uint64_t n = 0;
-    for (auto &p : r) n += p.n;
-    max_round = std::max(max_round, n);
-  }
-  if ((rc = ensure_seed_cap(e, (uint32_t)std::min<uint64_t>(max_round, 0xFFFFFFFFull)))) return rc;
+  if (P)
+    HIPCHK(hipMemcpyAsync(e->h_counts.p + (size_t)P->slot * nq * 2, e->counts.p + (size_t)P->slot * nq * 2,
+                          (size_t)nq * 2 * 8, hipMemcpyDeviceToHost, e->stream));
+  if (R && d) host_delta(e, R->ch.data(), (uint32_t)R->ch.size(), *d);
   HIPCHK(hipStreamSynchronize(e->stream));
-  HIPCHK(sv->h_rec.ensure(total + 2));
-  HIPCHK(sv->d_rec.ensure(total + 2));
-  {
-    size_t o = 0;
-    auto put = [&](const std::vector<uint32_t> &s, const std::vector<uint32_t> &d) {
-      if (s.empty()) return;
-      std::copy(s.begin(), s.end(), sv->h_rec.p + o);
-      std::copy(d.begin(), d.end(), sv->h_rec.p + o + s.size());
-      o += 2 * s.size();
-    };
-    put(ns[0], nd[0]);
-    for (uint32_t i = 0; i < m; i++) {
-      put(ps[i], pd[i]);
-      if (i + 1 < m) put(ns[i + 1], nd[i + 1]);
-    }
+  if (np + nn) {
+    const DfsCtl &c = *e->h_ctl.p;
+    if (c.seed_overflow.v) return e->fail_msg(DSM_ELIMIT, "seed table overflow in a served round");
+    if (c.abort.v)
+      return e->fail_msg(DSM_EDEVICE, "DFS watchdog: a wave waited more than %u ms for work", e->watchdog_ms);
   }
-  HIPCHK(hipMemcpyAsync(sv->d_rec.p, sv->h_rec.p, (total + 2) * 4, hipMemcpyHostToDevice, e->stream));
-  HIPCHK(sv->h_ctls.ensure(m + 1));
-  while (sv->ev_done.size() < m + 1) {
-    hipEvent_t ev;
-    HIPCHK(hipEventCreate(&ev));
-    sv->ev_done.push_back(ev);
-  }
-  if (!sv->ev_start) HIPCHK(hipEventCreate(&sv->ev_start));
-  HIPCHK(launch_ctl_reset(nullptr, nullptr, nullptr, e->counts.p, m * nq * 2, e->stream));
-  HIPCHK(hipStreamSynchronize(e->stream));
-  const double t0 = now_s();
-  HIPCHK(hipEventRecord(sv->ev_start, e->stream));
-  auto run = [&](uint32_t r) -> int {
-    int rc2;
-    if (rounds[r].empty()) {
-      HIPCHK(hipEventRecord(sv->ev_done[r], e->stream));
-      return DSM_OK;
-    }
-    if ((rc2 = round_begin(e))) return rc2;
-    uint32_t n = 0;
-    for (auto &p : rounds[r]) {
-      if ((rc2 = round_seed(e, sv->d_rec.p + p.off, sv->d_rec.p + p.off + p.n, p.n, p.tag))) return rc2;
-      n += p.n;
-    }
-    if ((rc2 = round_dfs(e, n))) return rc2;
-    HIPCHK(hipMemcpyAsync(sv->h_ctls.p + r, e->ctl.p, sizeof(DfsCtl), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipEventRecord(sv->ev_done[r], e->stream));
-    return DSM_OK;
-  };
-  if ((rc = run(0))) return rc;
-  for (uint32_t i = 0; i < m; i++) {
-    if ((rc = apply_and_refresh(e, G[i]->ch.data(), (uint32_t)G[i]->ch.size()))) return rc;
-    if ((rc = run(i + 1))) return rc;
-  }
-  HIPCHK(hipMemcpyAsync(e->h_counts.p, e->counts.p, (size_t)m * nq * 2 * 8, hipMemcpyDeviceToHost, e->stream));
-  HIPCHK(hipStreamSynchronize(e->stream));
-  for (uint32_t r = 0; r <= m; r++) {
-    if (rounds[r].empty()) continue;
-    const DfsCtl &c = sv->h_ctls.p[r];
-    if (c.seed_overflow.v) return e->fail_msg(DSM_ELIMIT, "seed table overflow in a served step");
-    if (c.abort.v) return e->fail_msg(DSM_EDEVICE, "DFS watchdog expired in a served step");
-  }
-  for (uint32_t i = 0; i < m; i++) {
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, sv->ev_start, sv->ev_done[i + 1]));
-    G[i]->t_done = t0 + ms * 1e-3;
-    G[i]->pos.assign(nq, 0);
-    G[i]->neg.assign(nq, 0);
+  if (P) {
+    P->pos.assign(nq, 0);
+    P->neg.assign(nq, 0);
+    const unsigned long long *c = e->h_counts.p + (size_t)P->slot * nq * 2;
     for (uint32_t q = 0; q < nq; q++) {
-      G[i]->pos[q] = e->h_counts.p[((size_t)i * nq + q) * 2 + 1];
-      G[i]->neg[q] = e->h_counts.p[((size_t)i * nq + q) * 2 + 0];
+      P->pos[q] = c[2 * q + 1];
+      P->neg[q] = c[2 * q + 0];
     }
   }
   return DSM_OK;
 }
 
 void server_loop(dsm_server *sv) {
+  dsm_engine *e = sv->e;
+  if (hipSetDevice(e->opt.device) != hipSuccess) sv->failed = DSM_EDEVICE;
+  ServerReq *P = nullptr;  // applied, positives not yet matched
   for (;;) {
-    std::vector<ServerReq *> G;
+    ServerReq *R = nullptr;
     {
       std::unique_lock<std::mutex> lk(sv->mu);
-      sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
-      if (sv->pending.empty()) return;  // stop requested and drained
-      std::vector<uint64_t> bytes;
-      for (size_t i = 0; i < sv->pending.size() && i < sv->max_per_step; i++) bytes.push_back(sv->pending[i]->bytes);
-      const uint32_t n = dsm_admit_count(bytes.data(), (uint32_t)bytes.size(), sv->budget, sv->max_per_step);
-      if (n == 0) {
-        // too large even alone: rejected without touching the graph
-        ServerReq *r = sv->pending.front();
+      if (!P) sv->cv_work.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
+      while (!sv->pending.empty()) {
+        ServerReq *c = sv->pending.front();
+        const int adm = sv->failed != DSM_OK ? sv->failed : dsm_admit(c->bytes, P ? P->bytes : 0, sv->budget);
+        if (adm < 0) {
+          // rejected without touching the graph
+          sv->pending.pop_front();
+          c->status = adm;
+          c->t_start = c->t_done = now_s();
+          sv->cv_done.notify_all();
+          continue;
+        }
+        if (adm == 0) break;  // waits until the request in flight is done
         sv->pending.pop_front();
-        r->status = DSM_EADMIT;
-        r->t_start = r->t_done = now_s();
-        sv->cv_done.notify_all();
+        R = c;
+        R->queue_depth = (uint32_t)sv->pending.size();
+        break;
+      }
+      if (!P && !R) {
+        if (sv->stop && sv->pending.empty()) return;
         continue;
       }
-      for (uint32_t i = 0; i < n; i++) {
-        G.push_back(sv->pending.front());
-        sv->pending.pop_front();
+    }
+    if (R) {
+      R->t_start = now_s();
+      R->slot = P ? P->slot ^ 1u : 0u;
+      R->fused = P ? 1u : 0u;
+      for (const dsm_change &c : R->ch) {
+        if (c.add) {
+          R->ps.push_back(c.src);
+          R->pd.push_back(c.dst);
+        } else if (e->opt.count_negatives) {
+          R->ns.push_back(c.src);
+          R->nd.push_back(c.dst);
+        }
       }
     }
-    const double ts = now_s();
-    const uint32_t step = sv->steps++;
-    for (auto *r : G) {
-      r->t_start = ts;
-      r->step = step;
-      r->step_size = (uint32_t)G.size();
-    }
-    const int rc = serve_step(sv, G);
-    {
+    HostDelta d;
+    int rc = serve_round(sv, P, R, R ? &d : nullptr);
+    if (P) publish(sv, P, rc);
+    if (rc == DSM_OK && R) rc = apply_delta_and_refresh(e, d);
+    if (rc != DSM_OK) {
       std::lock_guard<std::mutex> lk(sv->mu);
-      if (rc != DSM_OK) sv->err = sv->e->err;
-      for (auto *r : G) {
-        r->status = rc;
-        if (rc != DSM_OK) r->t_done = now_s();
-      }
+      sv->failed = rc;
+      sv->err = e->err;
     }
-    sv->cv_done.notify_all();
+    if (R && rc != DSM_OK) {
+      publish(sv, R, rc);
+      R = nullptr;
+    }
+    P = R;
   }
 }
 
@@ -1276,17 +1236,9 @@ void server_loop(dsm_server *sv) {
 
 extern "C" {
 
-uint32_t dsm_admit_count(const uint64_t *request_bytes, uint32_t num_pending, uint64_t budget,
-                         uint32_t max_per_step) {
-  uint64_t used = 0;
-  uint32_t n = 0;
-  if (max_per_step == 0 || max_per_step > kMaxSlots) max_per_step = kMaxSlots;
-  while (n < num_pending && n < max_per_step) {
-    if (request_bytes[n] > budget - used) break;
-    used += request_bytes[n];
-    n++;
-  }
-  return n;
+int dsm_admit(uint64_t request_bytes, uint64_t in_flight_bytes, uint64_t budget) {
+  if (request_bytes > budget) return DSM_EADMIT;
+  return in_flight_bytes <= budget - request_bytes ? 1 : 0;
 }
 
 uint64_t dsm_estimate_request_bytes(const dsm_engine *e, uint32_t num_changes) {
@@ -1298,6 +1250,7 @@ int dsm_server_create(dsm_engine *e, const dsm_server_options *opt, dsm_server *
   *out = nullptr;
   if (e->tuple_cap) return e->fail_msg(DSM_EINVAL, "served mode keeps counts only (tuple_capacity must be 0)");
   HIPCHK(hipSetDevice(e->opt.device));
+  HIPCHK(hipStreamSynchronize(e->stream));
   std::unique_ptr<dsm_server> sv(new dsm_server());
   sv->e = e;
   size_t freeb = 0, totalb = 0;
@@ -1306,7 +1259,6 @@ int dsm_server_create(dsm_engine *e, const dsm_server_options *opt, dsm_server *
   const double reserve = headroom * (double)totalb;
   if (opt && opt->budget_bytes) sv->budget = opt->budget_bytes;
   else sv->budget = (double)freeb > reserve ? (uint64_t)((double)freeb - reserve) : 0;
-  if (opt && opt->max_requests_per_step) sv->max_per_step = std::min<uint32_t>(opt->max_requests_per_step, kMaxSlots);
   dsm_server *p = sv.get();
   p->worker = std::thread(server_loop, p);
   *out = sv.release();
@@ -1351,8 +1303,8 @@ int dsm_server_wait(dsm_server *sv, uint64_t id, dsm_request_info *info, uint64_
     info->t_submit = r->t_submit;
     info->t_start = r->t_start;
     info->t_done = r->t_done;
-    info->step = r->step;
-    info->step_size = r->step_size;
+    info->queue_depth = r->queue_depth;
+    info->fused = r->fused;
     info->num_changes = (uint32_t)r->ch.size();
   }
   for (uint32_t q = 0; q < nq; q++) {
@@ -1375,8 +1327,6 @@ void dsm_server_destroy(dsm_server *sv) {
   sv->cv_work.notify_all();
   if (sv->worker.joinable()) sv->worker.join();
   for (auto &kv : sv->reqs) delete kv.second;
-  for (auto ev : sv->ev_done) (void)hipEventDestroy(ev);
-  if (sv->ev_start) (void)hipEventDestroy(sv->ev_start);
   delete sv;
 }
 

@@ -37,7 +37,7 @@ EXPORTS = (
     "dsm_get_candidates", "dsm_num_vertices", "dsm_num_queries", "dsm_file_read", "dsm_file_free",
     "dsm_file_graph", "dsm_file_query", "dsm_file_num_batches", "dsm_file_batch", "dsm_file_error",
     "dsm_server_create", "dsm_server_submit", "dsm_server_wait", "dsm_server_error", "dsm_server_destroy",
-    "dsm_admit_count", "dsm_estimate_request_bytes",
+    "dsm_admit", "dsm_estimate_request_bytes",
 )
 
 
@@ -82,13 +82,12 @@ class Stats(C.Structure):
 
 
 class ServerOptions(C.Structure):
-    _fields_ = [("budget_bytes", C.c_uint64), ("hbm_headroom", C.c_double),
-                ("max_requests_per_step", C.c_uint32)]
+    _fields_ = [("budget_bytes", C.c_uint64), ("hbm_headroom", C.c_double)]
 
 
 class RequestInfo(C.Structure):
     _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("t_submit", C.c_double), ("t_start", C.c_double),
-                ("t_done", C.c_double), ("step", C.c_uint32), ("step_size", C.c_uint32),
+                ("t_done", C.c_double), ("queue_depth", C.c_uint32), ("fused", C.c_uint32),
                 ("num_changes", C.c_uint32)]
 
     def as_dict(self):
@@ -156,8 +155,8 @@ def lib():
     L.dsm_server_error.restype = C.c_char_p
     L.dsm_server_error.argtypes = [vp]
     L.dsm_server_destroy.argtypes = [vp]
-    L.dsm_admit_count.restype = C.c_uint32
-    L.dsm_admit_count.argtypes = [vp, C.c_uint32, C.c_uint64, C.c_uint32]
+    L.dsm_admit.restype = C.c_int
+    L.dsm_admit.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
     L.dsm_estimate_request_bytes.restype = C.c_uint64
     L.dsm_estimate_request_bytes.argtypes = [vp, C.c_uint32]
     _LIB = L
@@ -336,10 +335,9 @@ class Engine:
         return bits[: self.n].astype(bool)
 
 
-def admit_count(request_bytes, budget, max_per_step=0):
-    """The server's admission rule (dsm_admit_count) on a list of request sizes."""
-    b = np.ascontiguousarray(np.asarray(request_bytes, np.uint64))
-    return int(lib().dsm_admit_count(b.ctypes.data, len(b), int(budget), int(max_per_step)))
+def admit(request_bytes, in_flight_bytes, budget):
+    """The server's admission rule (dsm_admit): 1 admit, 0 wait, DSM_EADMIT."""
+    return int(lib().dsm_admit(int(request_bytes), int(in_flight_bytes), int(budget)))
 
 
 class Server:
@@ -347,8 +345,8 @@ class Server:
     any thread, wait for each one's counts. The engine is not used directly
     while the server exists."""
 
-    def __init__(self, engine, budget_bytes=0, hbm_headroom=0.0, max_requests_per_step=0):
-        o = ServerOptions(int(budget_bytes), float(hbm_headroom), int(max_requests_per_step))
+    def __init__(self, engine, budget_bytes=0, hbm_headroom=0.0):
+        o = ServerOptions(int(budget_bytes), float(hbm_headroom))
         h = C.c_void_p()
         rc = lib().dsm_server_create(engine._h, C.byref(o), C.byref(h))
         if rc != DSM_OK:

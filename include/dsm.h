@@ -171,33 +171,36 @@ uint32_t dsm_num_queries(const dsm_engine *e);
 /* --------------------------------------------------------- served stream */
 
 /* A server runs a worker thread over one engine and serves update batches
- * ("requests") submitted from any thread, in submission order, with the same
- * per-batch semantics as dsm_process_batch. It forms steps out of the pending
- * requests: a step admits the oldest requests while their estimated
- * intermediate memory fits the budget (dsm_admit_count) and at most 64 of
- * them; one that does not fit even alone is rejected with DSM_EADMIT. Inside a
- * step, request i's positives and request i+1's negatives share one DFS
- * launch, and the host prepares each update while the previous launch runs.
- * No counterpart in the reference (one batch at a time, main.cu:152). While a
- * server exists, the engine must not be used directly. */
+ * ("requests") submitted from any thread, in submission order, each with the
+ * semantics of dsm_process_batch. It keeps a rolling pipeline: the positives
+ * of the request in flight (already applied) and the negatives of the next
+ * one are matched in one DFS launch over the same graph, the next request's
+ * delta is computed on the host while that launch runs, and each request's
+ * counts are published as soon as its positives are done. A request is
+ * admitted when it fits the memory budget beside the request in flight
+ * (dsm_admit); one that does not fit even alone is rejected with DSM_EADMIT.
+ * No counterpart in the reference, which runs one batch at a time
+ * (main.cu:152-348). While a server exists, the engine must not be used
+ * directly. */
 typedef struct dsm_server dsm_server;
 
 typedef struct {
   uint64_t budget_bytes;          /* admission budget (0: free HBM minus the headroom) */
   double hbm_headroom;            /* fraction of total HBM kept free (0: 0.30) */
-  uint32_t max_requests_per_step; /* 0: 64 */
 } dsm_server_options;
 
 typedef struct {
   uint64_t id;
   int status;                     /* DSM_OK, DSM_EADMIT or another error */
-  double t_submit, t_start, t_done; /* steady-clock seconds; t_done = its last DFS finished */
-  uint32_t step, step_size;       /* the step that served it and how many it served */
+  double t_submit, t_start, t_done; /* steady-clock seconds: submitted, admitted, counts published */
+  uint32_t queue_depth;           /* requests still pending when it was admitted */
+  uint32_t fused;                 /* 1: its negatives shared a launch with the previous request's positives */
   uint32_t num_changes;
 } dsm_request_info;
 
 int dsm_server_create(dsm_engine *e, const dsm_server_options *opt, dsm_server **out);
-/* Copies the changes and queues the request; *id identifies it. */
+/* Copies the changes and queues the request; *id identifies it. An
+ * out-of-range endpoint rejects it here with DSM_EINVAL. */
 int dsm_server_submit(dsm_server *s, const dsm_change *changes, uint32_t num_changes, uint64_t *id);
 /* Blocks until request id is done; fills its info and per-query counts (either
  * may be NULL) and returns its status. Each id can be waited for once. */
@@ -207,12 +210,10 @@ const char *dsm_server_error(const dsm_server *s);
 /* Serves what is pending, then stops the worker. */
 void dsm_server_destroy(dsm_server *s);
 
-/* Admission rule of the server: how many of the oldest pending requests
- * (estimated bytes request_bytes[0..num_pending)) form the next step under the
- * budget, at most max_per_step (0 or > 64: 64). 0 means the oldest request
- * does not fit even alone. Pure host logic. */
-uint32_t dsm_admit_count(const uint64_t *request_bytes, uint32_t num_pending, uint64_t budget,
-                         uint32_t max_per_step);
+/* Admission rule of the server, pure host logic: 1 = admit now (it fits
+ * beside the in-flight bytes), 0 = wait for the request in flight,
+ * DSM_EADMIT = larger than the whole budget. */
+int dsm_admit(uint64_t request_bytes, uint64_t in_flight_bytes, uint64_t budget);
 /* Estimated intermediate device memory of a request of num_changes records
  * (seed table, device records, update arrays). */
 uint64_t dsm_estimate_request_bytes(const dsm_engine *e, uint32_t num_changes);

@@ -1,6 +1,6 @@
 """Served stream (dsm_server_*): continuous batching of update requests.
 
-Host tests cover the admission rule (dsm_admit_count) without a GPU. GPU tests
+Host tests cover the admission rule (dsm_admit) without a GPU. GPU tests
 check that every request's counts are the same whether it is served alone
 (dsm_process_batch on an engine of its own, in the same order) or in the mix
 of a step, bit-exact, and against the CPU oracle on small inputs; and that
@@ -22,23 +22,17 @@ def say(msg):
 
 # ------------------------------------------------------------ host (no GPU)
 
-def test_admit_count_prefix_under_budget():
-    assert E.admit_count([10, 20, 30], 60) == 3
-    assert E.admit_count([10, 20, 30], 59) == 2
-    assert E.admit_count([10, 20, 30], 30) == 2
-    assert E.admit_count([10, 20, 30], 29) == 1
-    # the oldest request alone does not fit: nothing is admitted
-    assert E.admit_count([100, 1, 1], 99) == 0
-    # strictly a prefix: a small request behind a large one waits
-    assert E.admit_count([10, 100, 1], 50) == 1
-    assert E.admit_count([], 1000) == 0
-
-
-def test_admit_count_slot_limits():
-    assert E.admit_count([1] * 100, 10 ** 9) == 64
-    assert E.admit_count([1] * 100, 10 ** 9, max_per_step=5) == 5
-    assert E.admit_count([1] * 100, 10 ** 9, max_per_step=1000) == 64
-    assert E.admit_count([0] * 3, 0) == 3
+def test_admit_rule():
+    assert E.admit(10, 0, 10) == 1
+    assert E.admit(0, 0, 0) == 1
+    assert E.admit(5, 5, 10) == 1
+    # does not fit beside the request in flight: waits
+    assert E.admit(5, 6, 10) == 0
+    assert E.admit(10, 1, 10) == 0
+    # larger than the whole budget: rejected
+    assert E.admit(11, 0, 10) == E.DSM_EADMIT
+    assert E.admit(2 ** 63, 0, 2 ** 62) == E.DSM_EADMIT
+    assert E.admit(2 ** 62, 2 ** 62, 2 ** 63) == 1
 
 
 # ---------------------------------------------------------------- GPU tests
@@ -124,7 +118,8 @@ def test_served_equals_alone_in_order():
                 ids.append(srv.submit(*r))
             time.sleep(0.05)
         got = [srv.wait(i) for i in ids]
-        say("  served; step sizes %s" % sorted(set(g[3]["step_size"] for g in got)))
+        say("  served; %d of %d requests fused with their predecessor"
+            % (sum(g[3]["fused"] for g in got), len(got)))
         for i, r in enumerate(reqs):
             pa, na = seq.process_batch(*r)
             st, ps, ns, info = got[i]
@@ -132,7 +127,7 @@ def test_served_equals_alone_in_order():
             assert ps.tolist() == pa.tolist(), "request %d positives" % i
             assert ns.tolist() == na.tolist(), "request %d negatives" % i
             assert info["t_submit"] <= info["t_start"] <= info["t_done"]
-        assert max(g[3]["step_size"] for g in got) > 1, "no step served more than one request"
+        assert any(g[3]["fused"] for g in got), "no request shared a launch with its predecessor"
     finally:
         srv.close()
         srv_eng.close()
@@ -167,35 +162,34 @@ def test_served_matches_oracle():
 @pytest.mark.gpu
 def test_admission_rejects_oversize_request():
     """With a budget of 300 records' estimated memory, a 1,000-record request is
-    rejected (DSM_EADMIT) and leaves the graph alone; steps never exceed the
-    budget; the rest equal the plain path without the rejected one."""
+    rejected (DSM_EADMIT) and leaves the graph alone; a request that does not
+    fit beside the one in flight waits for it (never shares its launch); the
+    rest equal the plain path without the rejected ones."""
     say("case: admission by estimated memory")
     rng = np.random.default_rng(1203)
     n, m, nl = 500, 4000, 2
     labels, eu, ev = random_graph(n, m, nl, rng)
     queries = [qtuple("triangle", [0, 1, 1]), qtuple("square", [0, 1, 0, 1])]
     present = set(zip(eu.tolist(), ev.tolist()))
-    sizes = [50, 120, 1000, 80, 200, 1000, 10, 60]
+    sizes = [50, 120, 1000, 80, 250, 100, 1000, 10, 60]
     reqs = [random_request(n, k, present, rng, nl) for k in sizes]
     seq = E.Engine((n, labels, eu, ev), queries)
     eng = E.Engine((n, labels, eu, ev), queries)
-    budget = 0
     srv = None
     try:
         est = lambda k: int(E.lib().dsm_estimate_request_bytes(eng._h, k))  # noqa: E731
         budget = est(300)
-        assert est(1000) > budget >= est(200) + est(80)
+        assert est(1000) > budget and est(80) + est(250) > budget and est(250) + est(100) > budget
         srv = E.Server(eng, budget_bytes=budget)
         ids = [srv.submit(*r) for r in reqs]
         got = [srv.wait(i) for i in ids]
-        per_step = {}
         for (st, ps, ns, info), k in zip(got, sizes):
             if k == 1000:
                 assert st == E.DSM_EADMIT
                 continue
             assert st == E.DSM_OK, srv.error()
-            per_step[info["step"]] = per_step.get(info["step"], 0) + est(k)
-        assert max(per_step.values()) <= budget
+            if k in (250, 100):
+                assert info["fused"] == 0, "request of %d records shared a launch over budget" % k
         for (st, ps, ns, _), r, k in zip(got, reqs, sizes):
             if k == 1000:
                 continue
