@@ -560,8 +560,15 @@ struct WaveHdr {
   uint8_t qv[kMaxQ];
   uint8_t lix[kMaxQ];
   uint32_t n, query, plan, tag;
+  uint32_t fstate, fn;       // leaf set F of the current positions 0..n-3 (see leaf_count)
 };
-constexpr uint32_t kWin = 256;  // window entries staged in LDS per intersection
+constexpr uint32_t kWin = 256;   // window entries staged in LDS per intersection
+constexpr uint32_t kFCap = 256;  // largest leaf set F kept in LDS
+// states of the leaf set F
+constexpr uint32_t kFStale = 0;  // not computed for the current positions
+constexpr uint32_t kFReady = 1;  // fbuf[0, fn) holds F
+constexpr uint32_t kFLarge = 2;  // the shortest fixed list exceeds kFCap: per-candidate descent
+constexpr uint32_t kFNone = 3;   // the leaf's only backward neighbour is position n-2
 
 struct WaveMem {
   WaveHdr *h;
@@ -569,48 +576,57 @@ struct WaveMem {
   uint32_t *cand;   // [levels][64]
   uint32_t *cst;    // [levels][64] list start of each candidate
   uint32_t *cdg;    // [levels][64] degree of each candidate
+  uint32_t *fbuf;   // [kFCap] leaf set F, ascending
 };
 
 __host__ __device__ constexpr uint32_t wave_lds_bytes(uint32_t chunk_levels) {
-  return (uint32_t)((sizeof(WaveHdr) + 15) / 16 * 16) + kWin * 4 + chunk_levels * 64 * 12;
+  return (uint32_t)((sizeof(WaveHdr) + 15) / 16 * 16) + kWin * 4 + kFCap * 4 + chunk_levels * 64 * 12;
 }
 
-// Chooses the level's source: among the backward neighbours' lists, the one
-// with the fewest entries of the level's label (for a vertex of degree >
-// kLabelSorted, its label bucket in the label-sorted copy; otherwise the whole
-// list). Lane j examines backward position j; a wave minimum picks the
-// shortest, lowest position on ties.
-__device__ __forceinline__ void init_level(WaveHdr &S, const DevGraph &g, uint32_t l, uint32_t lo, uint32_t hi) {
+// Among the lists of the positions in `mask`, the one with the fewest entries
+// of label index li: for a vertex of degree > kLabelSorted, its label bucket in
+// the label-sorted copy; otherwise the whole list. Lane j examines position j;
+// a wave minimum picks the shortest, lowest position on ties. Wave-uniform
+// results; an empty mask gives length 0.
+__device__ __forceinline__ void pick_source(const WaveHdr &S, const DevGraph &g, uint32_t mask, uint32_t li,
+                                            const uint32_t *&src, uint32_t &len, uint32_t &j) {
   const uint32_t lane = lane_id();
-  const uint32_t bm = S.back[l];
   uint64_t key = ~0ull;
   uint32_t off = 0, sorted = 0;
-  if (lane < l && (bm >> lane & 1)) {
+  if (lane < (uint32_t)kMaxQ && (mask >> lane & 1)) {
     const uint32_t d = S.fd[lane];
-    uint32_t len = d;
+    uint32_t n = d;
     off = S.fs[lane];
     if (d > kLabelSorted) {
       const uint32_t *row = g.loff + (size_t)S.f[lane] * (g.nbuckets + 1);
-      const uint32_t li = S.lix[l];
       const uint32_t b0 = row[li], b1 = row[li + 1];
-      len = b1 - b0;
+      n = b1 - b0;
       off += b0;
       sorted = 1;
     }
-    key = ((uint64_t)len << 32) | lane;
+    key = ((uint64_t)n << 32) | lane;
   }
   for (int o = 32; o >= 1; o >>= 1) {
     const uint64_t y = __shfl_xor(key, o);
     key = y < key ? y : key;
   }
-  const uint32_t bj = key == ~0ull ? 0u : (uint32_t)(key & 0xFFFFFFFFu);
-  const uint32_t best = key == ~0ull ? 0u : (uint32_t)(key >> 32);  // no backward neighbour: empty
-  const uint32_t boff = __shfl(off, (int)bj);
-  const uint32_t bsorted = __shfl(sorted, (int)bj);
+  j = key == ~0ull ? 0u : (uint32_t)(key & 0xFFFFFFFFu);
+  len = key == ~0ull ? 0u : (uint32_t)(key >> 32);
+  const uint32_t boff = __shfl(off, (int)j);
+  const uint32_t bsorted = __shfl(sorted, (int)j);
+  src = (bsorted ? g.lpool : g.pool) + boff;
+}
+
+// Sets up level l to scan source indices [lo, min(hi, length)) of the shortest
+// backward list (pick_source); no backward neighbour leaves it empty.
+__device__ __forceinline__ void init_level(WaveHdr &S, const DevGraph &g, uint32_t l, uint32_t lo, uint32_t hi) {
+  const uint32_t *src;
+  uint32_t best, bj;
+  pick_source(S, g, S.back[l], S.lix[l], src, best, bj);
   wave_sync();
-  if (lane == 0) {
+  if (lane_id() == 0) {
     S.srcj[l] = bj;
-    S.src[l] = (bsorted ? g.lpool : g.pool) + boff;
+    S.src[l] = src;
     S.pos[l] = lo;
     S.end[l] = hi < best ? hi : best;
     S.cn[l] = 0;
@@ -667,28 +683,26 @@ __device__ __forceinline__ bool lds_contains(const uint32_t *a, uint32_t n, uint
   return a[lo] == x;
 }
 
-// Tests source entries [p, min(p + 64, e)) of level l, one per lane: the
-// candidate bit of the level's signature, injectivity against earlier
-// positions of the same label, then membership in every other backward list.
-// The source chunk is sorted, so the entries still alive span [wmin, wmax];
-// the wave finds that window of the other list cooperatively (bounds2) and,
-// when it holds at most kWin entries, stages it in LDS for the lanes' searches.
-__device__ __forceinline__ bool test_chunk(const WaveMem &M, const DfsArgs &a, uint32_t l, uint32_t p,
-                                           uint32_t e, uint32_t &w, uint32_t &loads) {
+// Tests source entries [p, min(p + 64, e)) of `src`, one per lane: the
+// candidate bit of signature `sig`, injectivity against the positions in
+// `inj`, then membership in the list of every position in `oth`. The source
+// chunk is sorted, so the entries still alive span [wmin, wmax]; the wave
+// finds that window of the other list cooperatively (bounds2) and, when it
+// holds at most kWin entries, stages it in LDS for the lanes' searches.
+__device__ __forceinline__ bool test_entries(const WaveMem &M, const DfsArgs &a, const uint32_t *src, uint32_t oth,
+                                             uint32_t inj, uint32_t sig, uint32_t p, uint32_t e, uint32_t &w,
+                                             uint32_t &loads) {
   WaveHdr &S = *M.h;
   const uint32_t lane = lane_id();
   const uint32_t i = p + lane;
-  const uint32_t sj = S.srcj[l];
   bool ok = i < e;
-  w = ok ? S.src[l][i] : 0xFFFFFFFFu;
-  ok = ok && bit_of(a.bm, a.words, S.sig[l], w);
-  uint32_t inj = S.inj[l];
+  w = ok ? src[i] : 0xFFFFFFFFu;
+  ok = ok && bit_of(a.bm, a.words, sig, w);
   while (inj) {
     const uint32_t j = __builtin_ctz(inj);
     inj &= inj - 1;
     ok = ok && S.f[j] != w;
   }
-  uint32_t oth = S.back[l] & ~(1u << sj);
   while (oth) {
     const uint32_t j = __builtin_ctz(oth);
     oth &= oth - 1;
@@ -713,6 +727,171 @@ __device__ __forceinline__ bool test_chunk(const WaveMem &M, const DfsArgs &a, u
     }
   }
   return ok;
+}
+
+// The level's own chunk test: source = its chosen list, membership in the
+// other backward lists, injectivity and signature of level l.
+__device__ __forceinline__ bool test_chunk(const WaveMem &M, const DfsArgs &a, uint32_t l, uint32_t p,
+                                           uint32_t e, uint32_t &w, uint32_t &loads) {
+  const WaveHdr &S = *M.h;
+  return test_entries(M, a, S.src[l], S.back[l] & ~(1u << S.srcj[l]), S.inj[l], S.sig[l], p, e, w, loads);
+}
+
+// Leaf set F for positions 0..n-3 (leaf = n-1): the images the leaf may take
+// judging only by its backward neighbours other than position n-2, i.e. the
+// entries of the intersection of their lists (label bucket of the shortest)
+// that carry the leaf's signature bit and differ from the earlier images of
+// its label. F is sorted. Sets fstate: kFReady (F in fbuf), kFLarge (shortest
+// such list longer than kFCap) or kFNone (no such neighbour).
+__device__ void compute_leaf_set(const WaveMem &M, const DfsArgs &a, uint32_t leaf, uint32_t &scanned,
+                                 uint32_t &loads) {
+  WaveHdr &S = *M.h;
+  const uint32_t lane = lane_id();
+  const uint32_t pv = leaf - 1;
+  const uint32_t bp = S.back[leaf] & ~(1u << pv);
+  uint32_t state = kFReady, cnt = 0;
+  if (bp == 0) {
+    state = kFNone;
+  } else {
+    const uint32_t *src;
+    uint32_t len, j;
+    pick_source(S, a.g, bp, S.lix[leaf], src, len, j);
+    if (len > kFCap) {
+      state = kFLarge;
+    } else {
+      const uint32_t oth = bp & ~(1u << j);
+      const uint32_t inj = S.inj[leaf] & ~(1u << pv);
+      const uint32_t sig = S.sig[leaf];
+      for (uint32_t p = 0; p < len; p += 64) {
+        uint32_t w;
+        const bool ok = test_entries(M, a, src, oth, inj, sig, p, len, w, loads);
+        scanned += p + lane < len ? 1u : 0u;
+        const uint64_t m = ballot(ok);
+        if (ok) M.fbuf[cnt + mbcnt(m)] = w;
+        cnt += (uint32_t)__popcll(m);
+      }
+    }
+  }
+  wave_sync();
+  if (lane == 0) {
+    S.fstate = state;
+    S.fn = cnt;
+  }
+  wave_sync();
+}
+
+// Leaf images of the partial match whose position n-2 holds w (this lane's
+// candidate), given F. With the leaf not adjacent to position n-2 the count is
+// |F|. Otherwise it scans w's list (its label bucket when sorted copies exist):
+// with F, the entries that lie in F; with no other backward neighbour
+// (kFNone), the entries that carry the leaf's signature and differ from the
+// earlier images of its label. Injectivity against w itself applies when the
+// leaf has w's label. A lane whose scan would take more than kLaneWork steps
+// leaves it to the wave (leaf_count_wave) and reports its list instead.
+constexpr uint32_t kLaneWork = 32;
+
+struct LeafList {
+  const uint32_t *L;
+  uint32_t len;
+  bool by_f;  // search F's elements in L instead of scanning L
+};
+
+__device__ __forceinline__ bool leaf_accept(const WaveHdr &S, const DfsArgs &a, const WaveMem &M, uint32_t leaf,
+                                            uint32_t st, uint32_t fn, bool injw, uint32_t w, uint32_t y) {
+  if (injw && y == w) return false;
+  if (st == kFReady) return lds_contains(M.fbuf, fn, y);
+  if (!bit_of(a.bm, a.words, S.sig[leaf], y)) return false;
+  uint32_t m = S.inj[leaf] & ~(1u << (leaf - 1));
+  while (m) {
+    const uint32_t j = __builtin_ctz(m);
+    m &= m - 1;
+    if (S.f[j] == y) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ uint32_t leaf_count_lane(const WaveMem &M, const DfsArgs &a, uint32_t leaf, uint32_t w,
+                                                    bool &defer, LeafList &out, uint32_t &scanned,
+                                                    uint32_t &loads) {
+  const WaveHdr &S = *M.h;
+  const uint32_t pv = leaf - 1;
+  const uint32_t back = S.back[leaf];
+  const bool injw = (S.inj[leaf] >> pv) & 1;
+  const uint32_t st = S.fstate, fn = S.fn;
+  defer = false;
+  if (!((back >> pv) & 1)) {
+    uint32_t c = fn;
+    if (injw && lds_contains(M.fbuf, fn, w)) c--;
+    return c;
+  }
+  const uint32_t ws = a.g.start[w], wd = a.g.deg[w];
+  const uint32_t *L = a.g.pool + ws;
+  uint32_t len = wd;
+  if (wd > kLabelSorted) {
+    const uint32_t *row = a.g.loff + (size_t)w * (a.g.nbuckets + 1);
+    const uint32_t li = S.lix[leaf];
+    const uint32_t b0 = row[li], b1 = row[li + 1];
+    L = a.g.lpool + ws + b0;
+    len = b1 - b0;
+    loads += 2;
+  }
+  const uint32_t lg = 33u - (uint32_t)__builtin_clz(len | 1u);
+  const bool by_f = st == kFReady && (uint64_t)fn * lg < len;
+  const uint32_t work = by_f ? fn * lg : len;
+  if (work > kLaneWork) {
+    defer = true;
+    out.L = L;
+    out.len = len;
+    out.by_f = by_f;
+    return 0;
+  }
+  uint32_t c = 0;
+  if (by_f) {
+    for (uint32_t k = 0; k < fn; k++) {
+      const uint32_t x = M.fbuf[k];
+      c += (!(injw && x == w) && contains(L, len, x)) ? 1u : 0u;
+    }
+    loads += fn * lg;
+  } else {
+    for (uint32_t k = 0; k < len; k++) c += leaf_accept(S, a, M, leaf, st, fn, injw, w, L[k]) ? 1u : 0u;
+    scanned += len;
+  }
+  return c;
+}
+
+// The deferred lanes' lists, one after another, 64 entries (or F elements) at
+// a time across the wave; returns this lane's share of the count.
+__device__ __forceinline__ uint32_t leaf_count_wave(const WaveMem &M, const DfsArgs &a, uint32_t leaf, uint32_t w,
+                                                    bool defer, const LeafList &ll, uint32_t &scanned,
+                                                    uint32_t &loads) {
+  const WaveHdr &S = *M.h;
+  const uint32_t lane = lane_id();
+  const bool injw = (S.inj[leaf] >> (leaf - 1)) & 1;
+  const uint32_t st = S.fstate, fn = S.fn;
+  uint32_t c = 0;
+  uint64_t dm = ballot(defer);
+  while (dm) {
+    const int src = (int)__builtin_ctzll(dm);
+    dm &= dm - 1;
+    const uint32_t *L = (const uint32_t *)__shfl((unsigned long long)ll.L, src);
+    const uint32_t len = __shfl(ll.len, src);
+    const uint32_t ww = __shfl(w, src);
+    const bool by_f = __shfl((uint32_t)ll.by_f, src) != 0;
+    if (by_f) {
+      const uint32_t lg = 33u - (uint32_t)__builtin_clz(len | 1u);
+      for (uint32_t k = lane; k < fn; k += 64) {
+        const uint32_t x = M.fbuf[k];
+        c += (!(injw && x == ww) && contains(L, len, x)) ? 1u : 0u;
+        loads += lg;
+      }
+    } else {
+      for (uint32_t k = lane; k < len; k += 64) {
+        c += leaf_accept(S, a, M, leaf, st, fn, injw, ww, L[k]) ? 1u : 0u;
+        scanned++;
+      }
+    }
+  }
+  return c;
 }
 
 __device__ void emit_tuples(const WaveHdr &S, const DfsArgs &a, uint32_t l, uint32_t w, bool ok) {
@@ -753,7 +932,9 @@ __device__ void try_donate(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l
     const uint32_t p = S.pos[k], e = S.end[k];
     const uint32_t unf = e > p ? e - p : 0;
     if (k == l) {
-      if (unf >= a.donate_min) {
+      // entries one level above the leaf carry a leaf count each
+      const uint32_t dmin = (k + 2 == n && a.donate_min > 64) ? 64u : a.donate_min;
+      if (unf >= dmin) {
         lv = k;
         mid = p + unf / 2;
       }
@@ -832,6 +1013,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
     M.cand = M.win + kWin;
     M.cst = M.cand + chunk_levels * 64;
     M.cdg = M.cst + chunk_levels * 64;
+    M.fbuf = M.cdg + chunk_levels * 64;
   }
   WaveHdr &S = *M.h;
   if (lane == 0) S.plan = 0xFFFFFFFFu;
@@ -949,6 +1131,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
       S.fs[lane] = a.g.start[fw];
       S.fd[lane] = a.g.deg[fw];
     }
+    if (lane == 0) S.fstate = kFStale;
     wave_sync();
     const uint32_t n = bcast(S.n);
     uint64_t lcount = 0;
@@ -996,11 +1179,54 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
             S.f[l] = w;
             S.fs[l] = ws;
             S.fd[l] = wd;
+            // F depends on positions 0..n-3
+            if (l + 3 <= n) S.fstate = kFStale;
           }
           wave_sync();
           l++;
           init_level(S, a.g, l, 0, 0xFFFFFFFFu);
           continue;
+        }
+        if (!TUP && l + 2 == n && bcast(S.fstate) != kFLarge) {
+          // one level above the leaf: each lane counts the leaf images of its
+          // own candidate against the shared leaf set F, instead of a descent
+          // per candidate
+          bool large = false;
+          for (;;) {
+            if (++tick >= kDonateEvery) {
+              tick = 0;
+              try_donate(S, a, L0, l, n);
+            }
+            const uint32_t p = bcast(S.pos[l]), e = bcast(S.end[l]);
+            if (p >= e) break;
+            uint32_t w;
+            const bool ok = test_chunk(M, a, l, p, e, w, loads);
+            scanned += p + lane < e ? 1u : 0u;
+            if (ballot(ok) == 0) {
+              wave_sync();
+              if (lane == 0) S.pos[l] = p + 64;
+              wave_sync();
+              continue;
+            }
+            if (bcast(S.fstate) == kFStale) compute_leaf_set(M, a, l + 1, scanned, loads);
+            if (bcast(S.fstate) == kFLarge) {
+              // this chunk and the rest of the level take the descent path
+              large = true;
+              break;
+            }
+            bool defer = false;
+            LeafList ll{nullptr, 0, false};
+            if (ok) lcount += leaf_count_lane(M, a, l + 1, w, defer, ll, scanned, loads);
+            lcount += leaf_count_wave(M, a, l + 1, w, defer, ll, scanned, loads);
+            wave_sync();
+            if (lane == 0) S.pos[l] = p + 64;
+            wave_sync();
+          }
+          if (!large) {
+            if (l == L0) break;
+            l--;
+            continue;
+          }
         }
         if (++tick >= kDonateEvery) {
           tick = 0;

@@ -95,7 +95,8 @@ class RequestInfo(C.Structure):
 
 
 def library_path():
-    return os.path.join(_HERE, "libdsm_amd.so")
+    """The in-tree build; DSM_AMD_LIBRARY names another build of the same library."""
+    return os.environ.get("DSM_AMD_LIBRARY") or os.path.join(_HERE, "libdsm_amd.so")
 
 
 def lib():
