@@ -572,6 +572,7 @@ int check_round(dsm_engine *e, uint64_t *seeds) {
   e->stats.tasks_donated += std::min(c.q_tail.v, e->qcap);
   e->stats.dfs_entries += c.scanned.v;
   e->stats.dfs_search_loads += c.loads.v;
+  e->stats.dfs_merge_bytes += c.mbytes.v;
   e->stats.dfs_launches += 1;
   return DSM_OK;
 }

@@ -72,6 +72,7 @@ struct DfsCtl {
   CtlLine<uint32_t> abort;         // set when an idle wave outlived the watchdog
   CtlLine<unsigned long long> scanned;  // source-list entries tested
   CtlLine<unsigned long long> loads;    // 4-byte reads of membership searches
+  CtlLine<unsigned long long> mbytes;   // 4 B x degree of every backward list of every expanded partial match
 };
 
 // Status of the apply kernels of one batch.

@@ -520,6 +520,7 @@ __global__ void k_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tup
       ctl->abort.v = 0;
       ctl->scanned.v = 0;
       ctl->loads.v = 0;
+      ctl->mbytes.v = 0;
     }
     if (actl) {
       actl->scratch_used = 0;
@@ -581,6 +582,18 @@ struct WaveMem {
 
 __host__ __device__ constexpr uint32_t wave_lds_bytes(uint32_t chunk_levels) {
   return (uint32_t)((sizeof(WaveHdr) + 15) / 16 * 16) + kWin * 4 + kFCap * 4 + chunk_levels * 64 * 12;
+}
+
+// 4 B per entry of the full lists of the positions in `mask`: what a merge
+// intersection over those lists reads (the work measure of the expansion).
+__device__ __forceinline__ uint64_t list_bytes(const WaveHdr &S, uint32_t mask) {
+  uint64_t b = 0;
+  while (mask) {
+    const uint32_t j = __builtin_ctz(mask);
+    mask &= mask - 1;
+    b += 4ull * S.fd[j];
+  }
+  return b;
 }
 
 // Among the lists of the positions in `mask`, the one with the fewest entries
@@ -812,7 +825,7 @@ __device__ __forceinline__ bool leaf_accept(const WaveHdr &S, const DfsArgs &a, 
 
 __device__ __forceinline__ uint32_t leaf_count_lane(const WaveMem &M, const DfsArgs &a, uint32_t leaf, uint32_t w,
                                                     bool &defer, LeafList &out, uint32_t &scanned,
-                                                    uint32_t &loads) {
+                                                    uint32_t &loads, uint32_t &wdeg) {
   const WaveHdr &S = *M.h;
   const uint32_t pv = leaf - 1;
   const uint32_t back = S.back[leaf];
@@ -825,6 +838,7 @@ __device__ __forceinline__ uint32_t leaf_count_lane(const WaveMem &M, const DfsA
     return c;
   }
   const uint32_t ws = a.g.start[w], wd = a.g.deg[w];
+  wdeg = wd;
   const uint32_t *L = a.g.pool + ws;
   uint32_t len = wd;
   if (wd > kLabelSorted) {
@@ -1134,13 +1148,16 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
     if (lane == 0) S.fstate = kFStale;
     wave_sync();
     const uint32_t n = bcast(S.n);
-    uint64_t lcount = 0;
+    uint64_t lcount = 0, mbytes = 0;
     uint32_t scanned = 0, loads = 0;  // per lane: source entries read, search loads
     if (L0 >= n) {
       // complete embedding (two-vertex query seeded by its only edge)
       if (lane == 0) lcount = 1;
       if (TUP) emit_tuples(S, a, n - 1, S.f[n - 1], lane == 0);
     } else {
+      // a seed expands its partial match here; a donated task continues one
+      // whose expansion the donor counted
+      if (got == 2 && lane == 0) mbytes += list_bytes(S, S.back[L0]);
       init_level(S, a.g, L0, lo, hi);
       uint32_t l = L0;
       uint32_t tick = 0;
@@ -1184,6 +1201,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
           }
           wave_sync();
           l++;
+          mbytes += lane == 0 ? list_bytes(S, S.back[l]) : 0ull;
           init_level(S, a.g, l, 0, 0xFFFFFFFFu);
           continue;
         }
@@ -1209,6 +1227,9 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
               continue;
             }
             if (bcast(S.fstate) == kFStale) compute_leaf_set(M, a, l + 1, scanned, loads);
+            // backward lists of the leaf other than position n-2, read by a
+            // merge for every candidate of this level
+            const uint64_t fixed_leaf_bytes = list_bytes(S, S.back[l + 1] & ~(1u << l));
             if (bcast(S.fstate) == kFLarge) {
               // this chunk and the rest of the level take the descent path
               large = true;
@@ -1216,7 +1237,11 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
             }
             bool defer = false;
             LeafList ll{nullptr, 0, false};
-            if (ok) lcount += leaf_count_lane(M, a, l + 1, w, defer, ll, scanned, loads);
+            uint32_t wdeg = 0;
+            if (ok) {
+              lcount += leaf_count_lane(M, a, l + 1, w, defer, ll, scanned, loads, wdeg);
+              mbytes += fixed_leaf_bytes + 4ull * wdeg;
+            }
             lcount += leaf_count_wave(M, a, l + 1, w, defer, ll, scanned, loads);
             wave_sync();
             if (lane == 0) S.pos[l] = p + 64;
@@ -1267,11 +1292,12 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
     }
     // wave sum of the lane counts
     uint64_t tot = lcount;
-    uint64_t sc = scanned, ld = loads;
+    uint64_t sc = scanned, ld = loads, mb = mbytes;
     for (int off = 32; off >= 1; off >>= 1) {
       tot += __shfl_xor(tot, off);
       sc += __shfl_xor(sc, off);
       ld += __shfl_xor(ld, off);
+      mb += __shfl_xor(mb, off);
     }
     if (lane == 0) {
       const uint32_t tg = S.tag;
@@ -1279,6 +1305,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
         atomicAdd(&a.counts[((size_t)(tg >> 1) * a.nq + S.query) * 2 + (tg & 1)], (unsigned long long)tot);
       atomicAdd(&a.ctl->scanned.v, (unsigned long long)sc);
       atomicAdd(&a.ctl->loads.v, (unsigned long long)ld);
+      atomicAdd(&a.ctl->mbytes.v, (unsigned long long)mb);
       atomicSub(&a.ctl->work_left.v, 1);
     }
   }

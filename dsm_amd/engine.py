@@ -75,7 +75,8 @@ class Stats(C.Structure):
                 ("relocated_vertices", C.c_uint32), ("ms_apply", C.c_float), ("ms_filter", C.c_float),
                 ("ms_seed", C.c_float), ("ms_dfs_pos", C.c_float), ("ms_dfs_neg", C.c_float),
                 ("pool_entries_used", C.c_uint64), ("pool_entries_capacity", C.c_uint64),
-                ("dfs_entries", C.c_uint64), ("dfs_search_loads", C.c_uint64), ("dfs_launches", C.c_uint32)]
+                ("dfs_entries", C.c_uint64), ("dfs_search_loads", C.c_uint64), ("dfs_launches", C.c_uint32),
+                ("dfs_merge_bytes", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -96,6 +96,9 @@ typedef struct {
    * candidate-bitmap word) and 4 B reads of the membership binary searches */
   uint64_t dfs_entries, dfs_search_loads;
   uint32_t dfs_launches;
+  /* 4 B x degree of every backward list of every partial match the DFS
+   * expanded: the bytes merge intersections over whole lists would read */
+  uint64_t dfs_merge_bytes;
 } dsm_stats;
 
 /* ------------------------------------------------------------ engine life */
