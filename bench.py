@@ -57,11 +57,24 @@ def qtuples(w):
     return [(q.n, q.labels, q.eu, q.ev) for q in w.queries]
 
 
-def run_batched(w, warmup, steps, engine_kw):
+def tune_sample(w, nb=4):
+    """Records of the first nb batches: matched read-only by the tuner, so the
+    deletes (edges of the current graph) seed DFS roots and the inserts, absent
+    until applied, seed none."""
+    parts = [w.batch(b) for b in range(min(nb, w.num_batches))]
+    return tuple(np.concatenate([p[i] for p in parts]) for i in range(4))
+
+
+def run_batched(w, warmup, steps, engine_kw, autotune=True):
     from dsm_amd import engine as E
     t0 = time.perf_counter()
     eng = E.Engine((w.n, w.labels, w.eu, w.ev), qtuples(w), **engine_kw)
     log("batched engine set-up %.2f s" % (time.perf_counter() - t0))
+    tuned = None
+    if autotune:
+        t0 = time.perf_counter()
+        tuned = eng.autotune(*tune_sample(w))
+        log("autotune %.2f s: %s" % (time.perf_counter() - t0, tuned))
     lat, recs = [], 0
     agg = {"ms_dfs": 0.0, "launches": 0, "entries": 0, "loads": 0, "merge_bytes": 0, "ms_apply": 0.0,
            "ms_filter": 0.0,
@@ -99,7 +112,8 @@ def run_batched(w, warmup, steps, engine_kw):
                                 st["seeds_pos"] + st["seeds_neg"], st["tasks_donated"]))
     finally:
         eng.close()
-    return {"lat": lat, "records": recs, "pos": pos_tot, "neg": neg_tot, "agg": agg, "per_batch": per_batch}
+    return {"lat": lat, "records": recs, "pos": pos_tot, "neg": neg_tot, "agg": agg, "per_batch": per_batch,
+            "tuned": tuned}
 
 
 def run_plain(w, warmup, steps, engine_kw):
@@ -128,7 +142,8 @@ def run_plain(w, warmup, steps, engine_kw):
     return {"lat": lat.tolist(), "records": recs, "counts": counts}
 
 
-def run_served(w, rates, first_batch, engine_kw, seed=7, max_requests=160, offered_s=6.0, hi=16384):
+def run_served(w, rates, first_batch, engine_kw, seed=7, max_requests=160, offered_s=6.0, hi=16384,
+               autotune=True):
     """Poisson arrivals of log-uniform-size requests at each rate in turn, on
     one server; the graph keeps evolving from one rate to the next."""
     import queue
@@ -137,6 +152,8 @@ def run_served(w, rates, first_batch, engine_kw, seed=7, max_requests=160, offer
     rng = np.random.default_rng(seed)
     mean_size = (hi - 1) / np.log(hi)
     eng = E.Engine((w.n, w.labels, w.eu, w.ev), qtuples(w), **engine_kw)
+    if autotune:
+        log("served engine autotune: %s" % eng.autotune(*tune_sample(w)))
     srv = E.Server(eng)
     cur = {"b": first_batch, "buf": None, "pos": 0}
 
@@ -267,10 +284,14 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--served-rates", default="1e4,3e4,5e4,7e4,1e5,3e5,1e6,1e7")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--waves-per-block", type=int, default=4)
     ap.add_argument("--donate-min", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--lane-work", type=int, default=0)
+    ap.add_argument("--leaf-sets", type=int, default=1)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run one D2D copy of this many bytes")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -285,9 +306,11 @@ def main():
     w = make_workload(args.workload)
     log("workload %s generated in %.1f s: %d vertices, %d initial edges, %d batches, %d queries"
         % (w.name, time.perf_counter() - t, w.n, len(w.eu), w.num_batches, len(w.queries)))
-    engine_kw = {"waves_per_block": args.waves_per_block, "donate_min": args.donate_min}
+    engine_kw = {"waves_per_block": args.waves_per_block, "donate_min": args.donate_min,
+                 "blocks_per_cu": args.blocks_per_cu, "lane_work": args.lane_work,
+                 "leaf_sets": bool(args.leaf_sets)}
 
-    bat = run_batched(w, args.warmup, args.steps, engine_kw)
+    bat = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune)
     total_s = sum(bat["lat"])
     value = bat["records"] / total_s
     ms_per_step = total_s / len(bat["lat"]) * 1e3
@@ -325,7 +348,7 @@ def main():
                                   "queries, batched (one persistent launch per graph state)"
                                   % (w.name, len(w.queries)),
                       "batch": 4096, "queries": len(w.queries), "mode": "batched",
-                      "parallelism": "single GPU"},
+                      "parallelism": "single GPU", "tuned": bat["tuned"]},
            "p50_batch_ms": round(pct(bat["lat"], 50) * 1e3, 3), "p99_batch_ms": round(pct(bat["lat"], 99) * 1e3, 3),
            "matches_per_s": round((bat["pos"] + bat["neg"]) / total_s, 1),
            "stages_ms_per_batch": {k: round(agg[k] / len(bat["lat"]), 3) for k in ("ms_apply", "ms_filter",
@@ -348,7 +371,7 @@ def main():
 
     if not args.no_served:
         rates = [float(x) for x in args.served_rates.split(",") if x]
-        sv = run_served(w, rates, args.warmup + args.steps, engine_kw)
+        sv = run_served(w, rates, args.warmup + args.steps, engine_kw, autotune=not args.no_autotune)
         out["served"] = sv
         best = max(sv, key=lambda r: r["value"])
         out["served_peak"] = {"value": best["value"], "offered": best["offered"], "p50_ms": best["p50_ms"],

@@ -134,6 +134,8 @@ struct dsm_engine {
   bool tuples_host = false;
   unsigned long long tuple_cap = 0;
   uint32_t dfs_blocks = 0, dfs_wpb = 4, max_qn = 2;
+  uint32_t donate_min = 256, lane_work = 16;
+  bool leaf_sets = true;
   uint32_t watchdog_ms = 60000;
   int wall_khz = 100000;
   // records of the last batch (file order)
@@ -533,7 +535,9 @@ int round_dfs(dsm_engine *e, uint32_t nrec_hint) {
   da.queue = e->queue.p;
   da.qcap = e->qcap;
   da.epoch = ++e->epoch;
-  da.donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
+  da.donate_min = e->donate_min;
+  da.lane_work = e->lane_work;
+  da.leaf_sets = e->leaf_sets ? 1u : 0u;
   da.nq = (uint32_t)e->queries.size();
   da.watchdog = (unsigned long long)e->wall_khz * e->watchdog_ms;
   da.ctl = e->ctl.p;
@@ -828,6 +832,9 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   int per_cu = dfs_max_blocks_per_cu(wpb, e->max_qn);
   if (e->opt.blocks_per_cu && (int)e->opt.blocks_per_cu < per_cu) per_cu = (int)e->opt.blocks_per_cu;
   e->dfs_blocks = (uint32_t)(per_cu * e->num_cu);
+  e->donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
+  e->lane_work = e->opt.lane_work ? e->opt.lane_work : 16;
+  e->leaf_sets = e->opt.leaf_sets != 0;
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
   HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -846,6 +853,7 @@ void dsm_options_default(dsm_options *opt) {
   opt->relabel = 1;
   opt->count_negatives = 1;
   opt->waves_per_block = 4;
+  opt->leaf_sets = 1;
 }
 
 const char *dsm_last_error(const dsm_engine *e) {
@@ -1041,6 +1049,149 @@ int dsm_get_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words) {
   HIPCHK(hipStreamSynchronize(e->stream));
   const uint32_t s = e->qsig[q][u];
   HIPCHK(hipMemcpy(words, e->bm.p + (size_t)s * e->words, (size_t)e->words * 8, hipMemcpyDeviceToHost));
+  return DSM_OK;
+}
+
+int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_report *rep) {
+  if (!e || (!sample && k)) return DSM_EINVAL;
+  if (e->batch_pending) return e->fail_msg(DSM_EINVAL, "a batch is applied but not matched yet");
+  for (uint32_t i = 0; i < k; i++)
+    if (sample[i].src >= e->n || sample[i].dst >= e->n)
+      return e->fail_msg(DSM_EINVAL, "sample record %u: endpoint out of range", i);
+  HIPCHK(hipSetDevice(e->opt.device));
+  int rc;
+  const uint32_t nq = (uint32_t)e->queries.size();
+  std::vector<uint32_t> s(k), d(k);
+  for (uint32_t i = 0; i < k; i++) {
+    s[i] = sample[i].src;
+    d[i] = sample[i].dst;
+  }
+  if ((rc = ensure_seed_cap(e, k))) return rc;
+  DevBuf<uint32_t> &rb = e->rec_neg;
+  if ((rc = upload_records(e, s, d, rb))) return rc;
+  struct Cfg {
+    uint32_t wpb, bpc, dmin, lw;
+    bool ls;
+  };
+  auto max_bpc = [&](uint32_t wpb) { return (uint32_t)std::max(1, dfs_max_blocks_per_cu(wpb, e->max_qn)); };
+  // times one sample round under c and checks its counts against the first
+  std::vector<unsigned long long> ref_counts;
+  uint32_t trials = 0;
+  auto timed = [&](const Cfg &c, float &ms) -> int {
+    e->dfs_wpb = c.wpb;
+    e->dfs_blocks = c.bpc * (uint32_t)e->num_cu;
+    e->donate_min = c.dmin;
+    e->lane_work = c.lw;
+    e->leaf_sets = c.ls;
+    HIPCHK(launch_ctl_reset(nullptr, nullptr, e->tuple_count.p, e->counts.p, nq * 2, e->stream));
+    int rc2 = run_round(e, rb.p, rb.p + k, k, 0, e->ev[9], e->ev[10], e->ev[11]);
+    if (rc2) return rc2;
+    HIPCHK(hipMemcpyAsync(e->h_counts.p, e->counts.p, (size_t)nq * 2 * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    uint64_t seeds = 0;
+    if ((rc2 = check_round(e, &seeds))) return rc2;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[10], e->ev[11]));
+    std::vector<unsigned long long> cnt(e->h_counts.p, e->h_counts.p + 2 * (size_t)nq);
+    if (ref_counts.empty()) ref_counts = cnt;
+    else if (cnt != ref_counts)
+      return e->fail_msg(DSM_EDEVICE, "autotune: launch configurations disagree on the counts");
+    return DSM_OK;
+  };
+  Cfg best{e->dfs_wpb, e->dfs_blocks / (uint32_t)e->num_cu, e->donate_min, e->lane_work, e->leaf_sets};
+  // warm-up: clocks and caches settle before anything is compared
+  float best_ms = 1e30f;
+  for (int r = 0; r < 4; r++) {
+    float t = 0;
+    if ((rc = timed(best, t))) return rc;
+    if (r >= 2) best_ms = std::min(best_ms, t);
+  }
+  const float ms0 = best_ms;
+  // A candidate is timed in three pairs interleaved with the incumbent (so a
+  // drift of the clocks hits both alike) and must beat it by 3% on the best
+  // of each.
+  auto consider = [&](const Cfg &c) -> int {
+    float tb = 1e30f, tc = 1e30f;
+    for (int r = 0; r < 3; r++) {
+      float t = 0;
+      int rc2 = timed(best, t);
+      if (rc2) return rc2;
+      tb = std::min(tb, t);
+      if ((rc2 = timed(c, t))) return rc2;
+      tc = std::min(tc, t);
+    }
+    trials++;
+    if (tc < 0.97f * tb) {
+      best = c;
+      best_ms = tc;
+    } else {
+      best_ms = std::min(best_ms, tb);
+    }
+    return DSM_OK;
+  };
+  // 1. waves per block (each at its occupancy limit); a tuple engine has one shape
+  if (!e->tuple_cap) {
+    const Cfg base = best;
+    for (uint32_t w : {1u, 2u, 4u, 8u}) {
+      if (w == base.wpb || w * dfs_lds_bytes_per_wave(e->max_qn) > 65536) continue;
+      Cfg c = base;
+      c.wpb = w;
+      c.bpc = max_bpc(w);
+      if ((rc = consider(c))) return rc;
+    }
+  }
+  // 2. blocks per CU below the occupancy limit
+  {
+    const Cfg base = best;
+    const uint32_t m = max_bpc(base.wpb);
+    for (uint32_t b : {m, (m * 3 + 3) / 4, (m + 1) / 2}) {
+      if (b == base.bpc || b == 0) continue;
+      Cfg c = base;
+      c.bpc = b;
+      if ((rc = consider(c))) return rc;
+    }
+  }
+  // 3. donation threshold
+  {
+    const Cfg base = best;
+    for (uint32_t v : {64u, 128u, 256u, 512u, 1024u}) {
+      if (v == base.dmin) continue;
+      Cfg c = base;
+      c.dmin = v;
+      if ((rc = consider(c))) return rc;
+    }
+  }
+  // 4. leaf sets, 5. lane work
+  if (!e->tuple_cap) {
+    Cfg c = best;
+    c.ls = !best.ls;
+    if ((rc = consider(c))) return rc;
+    if (best.ls) {
+      const Cfg base = best;
+      for (uint32_t v : {8u, 16u, 24u, 48u, 64u}) {
+        if (v == base.lw) continue;
+        Cfg c2 = base;
+        c2.lw = v;
+        if ((rc = consider(c2))) return rc;
+      }
+    }
+  }
+  // the kept configuration, timed once more
+  best_ms = 1e30f;
+  for (int r = 0; r < 2; r++) {
+    float t = 0;
+    if ((rc = timed(best, t))) return rc;
+    best_ms = std::min(best_ms, t);
+  }
+  if (rep) {
+    rep->waves_per_block = best.wpb;
+    rep->blocks_per_cu = best.bpc;
+    rep->donate_min = best.dmin;
+    rep->lane_work = best.lw;
+    rep->leaf_sets = best.ls ? 1 : 0;
+    rep->trials = trials;
+    rep->ms_initial = ms0;
+    rep->ms_best = best_ms;
+  }
   return DSM_OK;
 }
 

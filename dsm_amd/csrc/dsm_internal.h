@@ -141,6 +141,8 @@ struct DfsArgs {
   uint32_t qcap;
   uint32_t epoch;
   uint32_t donate_min;
+  uint32_t lane_work;            // longest leaf list one lane scans alone
+  uint32_t leaf_sets;            // fused leaf level on/off
   uint32_t nq;                   // queries (count layout)
   const uint32_t *seed_tag;      // per seed: count slot << 1 | sign
   unsigned long long watchdog;   // wall-clock ticks an idle wave may wait

@@ -799,9 +799,8 @@ __device__ void compute_leaf_set(const WaveMem &M, const DfsArgs &a, uint32_t le
 // with F, the entries that lie in F; with no other backward neighbour
 // (kFNone), the entries that carry the leaf's signature and differ from the
 // earlier images of its label. Injectivity against w itself applies when the
-// leaf has w's label. A lane whose scan would take more than kLaneWork steps
-// leaves it to the wave (leaf_count_wave) and reports its list instead.
-constexpr uint32_t kLaneWork = 32;
+// leaf has w's label. A lane whose scan would take more than a.lane_work
+// steps leaves it to the wave (leaf_count_wave) and reports its list instead.
 
 struct LeafList {
   const uint32_t *L;
@@ -852,7 +851,7 @@ __device__ __forceinline__ uint32_t leaf_count_lane(const WaveMem &M, const DfsA
   const uint32_t lg = 33u - (uint32_t)__builtin_clz(len | 1u);
   const bool by_f = st == kFReady && (uint64_t)fn * lg < len;
   const uint32_t work = by_f ? fn * lg : len;
-  if (work > kLaneWork) {
+  if (work > a.lane_work) {
     defer = true;
     out.L = L;
     out.len = len;
@@ -1205,7 +1204,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
           init_level(S, a.g, l, 0, 0xFFFFFFFFu);
           continue;
         }
-        if (!TUP && l + 2 == n && bcast(S.fstate) != kFLarge) {
+        if (!TUP && a.leaf_sets && l + 2 == n && bcast(S.fstate) != kFLarge) {
           // one level above the leaf: each lane counts the leaf images of its
           // own candidate against the shared leaf set F, instead of a descent
           // per candidate

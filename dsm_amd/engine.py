@@ -37,7 +37,7 @@ EXPORTS = (
     "dsm_get_candidates", "dsm_num_vertices", "dsm_num_queries", "dsm_file_read", "dsm_file_free",
     "dsm_file_graph", "dsm_file_query", "dsm_file_num_batches", "dsm_file_batch", "dsm_file_error",
     "dsm_server_create", "dsm_server_submit", "dsm_server_wait", "dsm_server_error", "dsm_server_destroy",
-    "dsm_admit", "dsm_estimate_request_bytes",
+    "dsm_admit", "dsm_estimate_request_bytes", "dsm_autotune",
 )
 
 
@@ -65,7 +65,17 @@ class Options(C.Structure):
     _fields_ = [("device", C.c_int), ("relabel", C.c_int), ("count_negatives", C.c_int),
                 ("tuple_capacity", C.c_uint64), ("tuples_in_host_memory", C.c_int),
                 ("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("donate_min", C.c_uint32), ("queue_capacity", C.c_uint32), ("watchdog_ms", C.c_uint32)]
+                ("donate_min", C.c_uint32), ("queue_capacity", C.c_uint32), ("watchdog_ms", C.c_uint32),
+                ("lane_work", C.c_uint32), ("leaf_sets", C.c_int)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("donate_min", C.c_uint32),
+                ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("trials", C.c_uint32),
+                ("ms_initial", C.c_float), ("ms_best", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Stats(C.Structure):
@@ -159,6 +169,8 @@ def lib():
     L.dsm_server_destroy.argtypes = [vp]
     L.dsm_admit.restype = C.c_int
     L.dsm_admit.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+    L.dsm_autotune.restype = C.c_int
+    L.dsm_autotune.argtypes = [vp, C.POINTER(Change), C.c_uint32, C.POINTER(TuneReport)]
     L.dsm_estimate_request_bytes.restype = C.c_uint64
     L.dsm_estimate_request_bytes.argtypes = [vp, C.c_uint32]
     _LIB = L
@@ -231,7 +243,7 @@ class Engine:
 
     def __init__(self, graph, queries, device=0, relabel=True, count_negatives=True, tuple_capacity=0,
                  tuples_in_host_memory=False, waves_per_block=4, blocks_per_cu=0, donate_min=0,
-                 queue_capacity=0, watchdog_ms=0):
+                 queue_capacity=0, watchdog_ms=0, lane_work=0, leaf_sets=True):
         L = lib()
         n, lab, eu, ev = graph
         self._keep = []
@@ -255,6 +267,8 @@ class Engine:
         opt.donate_min = donate_min
         opt.queue_capacity = queue_capacity
         opt.watchdog_ms = watchdog_ms
+        opt.lane_work = lane_work
+        opt.leaf_sets = int(bool(leaf_sets))
         h = C.c_void_p()
         rc = L.dsm_create(C.byref(gd), qds, len(queries), C.byref(opt), C.byref(h))
         self._keep = None
@@ -294,6 +308,14 @@ class Engine:
         self._check(lib().dsm_process_batch(self._h, rec.ctypes.data_as(C.POINTER(Change)), rec.shape[0],
                                             pos.ctypes.data, neg.ctypes.data))
         return pos, neg
+
+    def autotune(self, add, src, dst, label):
+        """dsm_autotune over these records; returns the report as a dict."""
+        rec = changes_array(add, src, dst, label)
+        rep = TuneReport()
+        self._check(lib().dsm_autotune(self._h, rec.ctypes.data_as(C.POINTER(Change)), rec.shape[0],
+                                       C.byref(rep)))
+        return rep.as_dict()
 
     def tuples(self, cap=None):
         """Tuples of the last batch: dict (query, sign) -> (N, n) uint32 array."""

@@ -79,6 +79,11 @@ typedef struct {
   uint32_t queue_capacity;   /* donated-task queue slots (0 = default) */
   uint32_t watchdog_ms;      /* an idle DFS wave that waits longer than this for
                                 work aborts the round with DSM_EDEVICE (0 = 60 s) */
+  uint32_t lane_work;        /* one level above the leaf: longest list a lane scans
+                                alone before the wave takes it over (0 = 16) */
+  int leaf_sets;             /* 1: one level above the leaf, candidates count their
+                                leaf images against a leaf set shared by the parent;
+                                0: one descent per candidate */
 } dsm_options;
 
 /* Per-batch statistics of the last dsm_apply_batch + dsm_match_batch. */
@@ -100,6 +105,15 @@ typedef struct {
    * expanded: the bytes merge intersections over whole lists would read */
   uint64_t dfs_merge_bytes;
 } dsm_stats;
+
+/* Outcome of dsm_autotune: the launch parameters kept and the DFS time of the
+ * sample round before and after. */
+typedef struct {
+  uint32_t waves_per_block, blocks_per_cu, donate_min, lane_work;
+  int leaf_sets;
+  uint32_t trials;        /* configurations timed */
+  float ms_initial, ms_best;
+} dsm_tune_report;
 
 /* ------------------------------------------------------------ engine life */
 
@@ -170,6 +184,16 @@ int dsm_get_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words);
 /* Number of vertices and queries of the engine. */
 uint32_t dsm_num_vertices(const dsm_engine *e);
 uint32_t dsm_num_queries(const dsm_engine *e);
+
+/* Chooses the DFS launch parameters for this engine's queries and graph:
+ * waves per block, blocks per CU, the donation threshold, the lane work limit
+ * and leaf sets on or off, by coordinate descent over timed DFS rounds that
+ * match the sample's records in the current graph (read-only, as negatives
+ * are). Every configuration must give the same counts (else DSM_EDEVICE).
+ * Keeps the fastest; changes no graph state but overwrites the counts and
+ * statistics of the last batch. Call between batches. No counterpart in the
+ * reference, whose launch shapes are constants (gpu_match.cu:1121-1431). */
+int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t num_changes, dsm_tune_report *report);
 
 /* --------------------------------------------------------- served stream */
 

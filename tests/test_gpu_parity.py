@@ -312,3 +312,72 @@ def test_tuples_in_pinned_host_memory():
     present = set(zip(eu.tolist(), ev.tolist()))
     batches = [conflict_free_batch(n, 50, present, rng) for _ in range(2)]
     compare_stream(n, labels, eu, ev, queries, batches, tuples=True, tuples_in_host_memory=True)
+
+
+def test_uint64_counts_beyond_32_bits_closed_form():
+    """All 19,900 edges of K_200 inserted in one batch into an empty graph,
+    4-clique query, one label: every ordered embedding (200*199*198*197) uses
+    six inserted edges and counts once per edge, so the positive count is
+    6 * 200*199*198*197 = 9,314,632,800 > 2^32 (closed form, no oracle)."""
+    say("case: K_200 batch, 4-clique, count beyond 32 bits")
+    n = 200
+    labels = np.zeros(n, np.uint32)
+    z = np.zeros(0, np.uint32)
+    iu, ju = np.triu_indices(n, 1)
+    k = len(iu)
+    batch = (np.ones(k, np.uint8), iu.astype(np.uint32), ju.astype(np.uint32), np.zeros(k, np.uint32))
+    eng = E.Engine((n, labels, z, z), [qtuple("clique4", [0, 0, 0, 0])])
+    try:
+        pos, neg = eng.process_batch(*batch)
+        want = 6 * 200 * 199 * 198 * 197
+        assert want > 2 ** 32
+        assert int(pos[0]) == want and int(neg[0]) == 0
+        # deleting them all again: the same count as negatives
+        pos, neg = eng.process_batch(np.zeros(k, np.uint8), batch[1], batch[2], batch[3])
+        assert int(neg[0]) == want and int(pos[0]) == 0
+    finally:
+        eng.close()
+
+
+def test_autotune_keeps_counts_and_state():
+    """dsm_autotune times launch configurations on a sample (every one must give
+    the same counts) and keeps the fastest; the stream that follows still
+    matches the oracle, graph and bitmaps included."""
+    say("case: autotune, then a stream against the oracle")
+    rng = np.random.default_rng(1111)
+    n, m, nl = 400, 5000, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple("house", [0, 1, 0, 1, 0]), qtuple("clique4", [0, 0, 1, 1]), qtuple("path6", [0, 1, 0, 1, 0, 1])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    sample = conflict_free_batch(n, 200, set(present), rng)
+    batches = [conflict_free_batch(n, 80, present, rng) for _ in range(3)]
+    g = O.Graph(n, labels, eu, ev)
+    oq = oracle_queries(queries)
+    eng = E.Engine((n, labels, eu, ev), queries)
+    try:
+        rep = eng.autotune(*sample)
+        say("  tuned: %s" % rep)
+        assert rep["waves_per_block"] in (1, 2, 4, 8) and rep["blocks_per_cu"] >= 1
+        assert rep["trials"] >= 5 and rep["ms_best"] > 0 and rep["ms_initial"] > 0
+        for bi, b in enumerate(batches):
+            want = O.run_stream(g, oq, [b])[0]
+            pos, neg = eng.process_batch(*b)
+            assert pos.tolist() == want["pos"] and neg.tolist() == want["neg"], "batch %d" % bi
+        off, nb, _ = eng.graph()
+        ooff, onb, _ = g.export()
+        assert off.tolist() == ooff.tolist() and np.array_equal(nb, onb)
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("leaf_sets,lane_work", [(False, 0), (True, 1), (True, 1000)])
+def test_leaf_level_strategies_agree(leaf_sets, lane_work):
+    say("case: leaf level leaf_sets=%s lane_work=%d" % (leaf_sets, lane_work))
+    rng = np.random.default_rng(1212)
+    n, m, nl = 300, 4000, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple("house", [0, 1, 0, 1, 0]), qtuple("clique4", [0, 0, 1, 1]), qtuple("square", [0, 1, 0, 1]),
+               qtuple("star4", [0, 1, 1, 0, 1]), qtuple("cycle6_chord", [0, 1, 0, 1, 0, 1])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batches = [conflict_free_batch(n, 60, present, rng) for _ in range(2)]
+    compare_stream(n, labels, eu, ev, queries, batches, relabel=True, leaf_sets=leaf_sets, lane_work=lane_work)
