@@ -136,6 +136,12 @@ struct dsm_engine {
   uint32_t dfs_blocks = 0, dfs_wpb = 4, max_qn = 2;
   uint32_t donate_min = 256, lane_work = 16;
   bool leaf_sets = true;
+  bool graph_rounds = true;
+  // the DFS round as a HIP graph (graph_round)
+  hipGraph_t rg_graph = nullptr;
+  hipGraphExec_t rg_exec = nullptr;
+  hipGraphNode_t rg_reset = nullptr, rg_seed0 = nullptr, rg_seed1 = nullptr, rg_dfs = nullptr;
+  uint32_t rg_key = 0;
   uint32_t watchdog_ms = 60000;
   int wall_khz = 100000;
   // records of the last batch (file order)
@@ -146,6 +152,8 @@ struct dsm_engine {
   dsm_stats stats{};
 
   ~dsm_engine() {
+    if (rg_exec) (void)hipGraphExecDestroy(rg_exec);
+    if (rg_graph) (void)hipGraphDestroy(rg_graph);
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (tuples) {
@@ -489,16 +497,16 @@ int ensure_seed_cap(dsm_engine *e, uint32_t nrec) {
   return DSM_OK;
 }
 
-// Seeds the records (device arrays rs/rd) and runs the persistent DFS.
-// A DFS round: reset the control block, seed one or more record lists (each
-// with its count tag, slot << 1 | sign), then one persistent DFS launch over
-// all their seeds. The seed table must hold the seeds of every list.
-int round_begin(dsm_engine *e) {
-  HIPCHK(launch_ctl_reset(e->ctl.p, nullptr, nullptr, nullptr, 0, e->stream));
-  return DSM_OK;
-}
+// A DFS round: reset the control block, seed up to two record lists (each
+// with its count tag, slot << 1 | sign), one persistent DFS launch over all
+// their seeds, and a copy of the control block to the host. The seed table
+// must hold the seeds of every list.
+struct RoundPart {
+  const uint32_t *rs, *rd;
+  uint32_t n, tag;
+};
 
-int round_seed(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nrec, uint32_t tag) {
+SeedArgs seed_args(dsm_engine *e, const RoundPart &p) {
   SeedArgs sa;
   sa.g = e->dev_graph();
   sa.bm = e->bm.p;
@@ -506,23 +514,20 @@ int round_seed(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t n
   sa.plans = e->d_plans.p;
   sa.plan_key = e->d_plan_key.p;
   sa.nplans = (uint32_t)e->plans.size();
-  sa.rs = rs;
-  sa.rd = rd;
-  sa.nrec = nrec;
+  sa.rs = p.rs;
+  sa.rd = p.rd;
+  sa.nrec = p.n;
   sa.seed_plan = e->seed_plan.p;
   sa.seed_s = e->seed_s.p;
   sa.seed_d = e->seed_d.p;
   sa.seed_tag = e->seed_tag.p;
   sa.seed_cap = e->seed_cap;
-  sa.tag = tag;
+  sa.tag = p.tag;
   sa.ctl = e->ctl.p;
-  HIPCHK(launch_seed(sa, e->stream));
-  return DSM_OK;
+  return sa;
 }
 
-// nrec_hint sizes the grid: a round over a handful of records gets a grid of
-// one block per CU, so that few waves pay the start-up and termination atomics.
-int round_dfs(dsm_engine *e, uint32_t nrec_hint) {
+DfsArgs dfs_args(dsm_engine *e) {
   DfsArgs da;
   da.g = e->dev_graph();
   da.bm = e->bm.p;
@@ -545,24 +550,87 @@ int round_dfs(dsm_engine *e, uint32_t nrec_hint) {
   da.tuples = e->tuples;
   da.tuple_cap = e->tuple_cap;
   da.tuple_count = e->tuple_count.p;
-  const uint64_t want = ((uint64_t)nrec_hint * std::max<uint32_t>(e->max_plans_per_key, 1) * 8 + e->dfs_wpb - 1) /
-                        e->dfs_wpb;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>(e->dfs_blocks, std::max<uint64_t>(want, (uint64_t)e->num_cu));
-  HIPCHK(launch_dfs(da, blocks, e->dfs_wpb, e->max_qn, e->stream));
-  HIPCHK(hipMemcpyAsync(e->h_ctl.p, e->ctl.p, sizeof(DfsCtl), hipMemcpyDeviceToHost, e->stream));
+  return da;
+}
+
+// The grid: a round over a handful of records gets one block per CU, so that
+// few waves pay the start-up and termination atomics.
+uint32_t dfs_grid(dsm_engine *e, uint32_t nrec) {
+  const uint64_t want =
+      ((uint64_t)nrec * std::max<uint32_t>(e->max_plans_per_key, 1) * 8 + e->dfs_wpb - 1) / e->dfs_wpb;
+  return (uint32_t)std::min<uint64_t>(e->dfs_blocks, std::max<uint64_t>(want, (uint64_t)e->num_cu));
+}
+
+void drop_round_graph(dsm_engine *e) {
+  if (e->rg_exec) (void)hipGraphExecDestroy(e->rg_exec);
+  if (e->rg_graph) (void)hipGraphDestroy(e->rg_graph);
+  e->rg_exec = nullptr;
+  e->rg_graph = nullptr;
+}
+
+// The round as one HIP graph: ctl reset -> seed list 0 -> seed list 1 -> DFS ->
+// control block to the host. Built once per launch shape (waves per block,
+// tuples or not); every later round updates the kernel nodes' arguments and
+// grids in the executable graph and launches it.
+int graph_round(dsm_engine *e, const RoundPart *parts, int nparts, uint32_t nrec) {
+  RoundNodeArgs st;
+  st.ctl = e->ctl.p;
+  const RoundPart none{nullptr, nullptr, 0, 0};
+  st.seed[0] = seed_args(e, nparts > 0 ? parts[0] : none);
+  st.seed[1] = seed_args(e, nparts > 1 ? parts[1] : none);
+  st.dfs = dfs_args(e);
+  hipKernelNodeParams pr, p0, p1, pd;
+  round_node_params(st, dfs_grid(e, nrec), e->dfs_wpb, e->max_qn, &pr, &p0, &p1, &pd);
+  const uint32_t key = (e->tuples ? 16u : 0u) | e->dfs_wpb;
+  if (!e->rg_exec || e->rg_key != key) {
+    drop_round_graph(e);
+    HIPCHK(hipGraphCreate(&e->rg_graph, 0));
+    HIPCHK(hipGraphAddKernelNode(&e->rg_reset, e->rg_graph, nullptr, 0, &pr));
+    HIPCHK(hipGraphAddKernelNode(&e->rg_seed0, e->rg_graph, &e->rg_reset, 1, &p0));
+    HIPCHK(hipGraphAddKernelNode(&e->rg_seed1, e->rg_graph, &e->rg_seed0, 1, &p1));
+    HIPCHK(hipGraphAddKernelNode(&e->rg_dfs, e->rg_graph, &e->rg_seed1, 1, &pd));
+    hipGraphNode_t cp;
+    HIPCHK(hipGraphAddMemcpyNode1D(&cp, e->rg_graph, &e->rg_dfs, 1, e->h_ctl.p, e->ctl.p, sizeof(DfsCtl),
+                                   hipMemcpyDeviceToHost));
+    HIPCHK(hipGraphInstantiate(&e->rg_exec, e->rg_graph, nullptr, nullptr, 0));
+    e->rg_key = key;
+  } else {
+    HIPCHK(hipGraphExecKernelNodeSetParams(e->rg_exec, e->rg_seed0, &p0));
+    HIPCHK(hipGraphExecKernelNodeSetParams(e->rg_exec, e->rg_seed1, &p1));
+    HIPCHK(hipGraphExecKernelNodeSetParams(e->rg_exec, e->rg_dfs, &pd));
+  }
+  HIPCHK(hipGraphLaunch(e->rg_exec, e->stream));
+  return DSM_OK;
+}
+
+// One round, eager or as a graph. ev0 / ev1 / ev2 (may be null) mark its start,
+// the end of seeding (eager only; a graph round records ev1 at its start) and
+// its end.
+int dfs_round(dsm_engine *e, const RoundPart *parts, int nparts, hipEvent_t ev0, hipEvent_t ev1,
+              hipEvent_t ev2) {
+  uint32_t nrec = 0;
+  for (int i = 0; i < nparts; i++) nrec += parts[i].n;
+  if (ev0) HIPCHK(hipEventRecord(ev0, e->stream));
+  if (e->graph_rounds) {
+    if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
+    int rc = graph_round(e, parts, nparts, nrec);
+    if (rc) return rc;
+  } else {
+    HIPCHK(launch_ctl_reset(e->ctl.p, nullptr, nullptr, nullptr, 0, e->stream));
+    for (int i = 0; i < nparts; i++)
+      if (parts[i].n) HIPCHK(launch_seed(seed_args(e, parts[i]), e->stream));
+    if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
+    HIPCHK(launch_dfs(dfs_args(e), dfs_grid(e, nrec), e->dfs_wpb, e->max_qn, e->stream));
+    HIPCHK(hipMemcpyAsync(e->h_ctl.p, e->ctl.p, sizeof(DfsCtl), hipMemcpyDeviceToHost, e->stream));
+  }
+  if (ev2) HIPCHK(hipEventRecord(ev2, e->stream));
   return DSM_OK;
 }
 
 int run_round(dsm_engine *e, const uint32_t *rs, const uint32_t *rd, uint32_t nrec, uint32_t tag,
               hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
-  int rc;
-  HIPCHK(hipEventRecord(ev0, e->stream));
-  if ((rc = round_begin(e))) return rc;
-  if ((rc = round_seed(e, rs, rd, nrec, tag))) return rc;
-  HIPCHK(hipEventRecord(ev1, e->stream));
-  if ((rc = round_dfs(e, nrec))) return rc;
-  HIPCHK(hipEventRecord(ev2, e->stream));
-  return DSM_OK;
+  const RoundPart p{rs, rd, nrec, tag};
+  return dfs_round(e, &p, 1, ev0, ev1, ev2);
 }
 
 int check_round(dsm_engine *e, uint64_t *seeds) {
@@ -835,6 +903,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   e->donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
   e->lane_work = e->opt.lane_work ? e->opt.lane_work : 16;
   e->leaf_sets = e->opt.leaf_sets != 0;
+  e->graph_rounds = e->opt.graph_rounds != 0;
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
   HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -854,6 +923,7 @@ void dsm_options_default(dsm_options *opt) {
   opt->count_negatives = 1;
   opt->waves_per_block = 4;
   opt->leaf_sets = 1;
+  opt->graph_rounds = 1;
 }
 
 const char *dsm_last_error(const dsm_engine *e) {
@@ -1294,10 +1364,11 @@ int serve_round(dsm_server *sv, ServerReq *P, ServerReq *R, HostDelta *d) {
       std::copy(R->nd.begin(), R->nd.end(), h + 2 * np + nn);
     }
     HIPCHK(hipMemcpyAsync(sv->d_rec.p, h, 2 * (size_t)(np + nn) * 4, hipMemcpyHostToDevice, e->stream));
-    if ((rc = round_begin(e))) return rc;
-    if (np && (rc = round_seed(e, sv->d_rec.p, sv->d_rec.p + np, np, (P->slot << 1) | 1u))) return rc;
-    if (nn && (rc = round_seed(e, sv->d_rec.p + 2 * np, sv->d_rec.p + 2 * np + nn, nn, R->slot << 1))) return rc;
-    if ((rc = round_dfs(e, np + nn))) return rc;
+    RoundPart parts[2];
+    int nparts = 0;
+    if (np) parts[nparts++] = RoundPart{sv->d_rec.p, sv->d_rec.p + np, np, (P->slot << 1) | 1u};
+    if (nn) parts[nparts++] = RoundPart{sv->d_rec.p + 2 * np, sv->d_rec.p + 2 * np + nn, nn, R->slot << 1};
+    if ((rc = dfs_round(e, parts, nparts, nullptr, nullptr, nullptr))) return rc;
   }
   if (P)
     HIPCHK(hipMemcpyAsync(e->h_counts.p + (size_t)P->slot * nq * 2, e->counts.p + (size_t)P->slot * nq * 2,

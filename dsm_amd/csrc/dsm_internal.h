@@ -210,6 +210,28 @@ hipError_t launch_pool_move(const uint32_t *old_pool, uint32_t *new_pool, const 
                             const uint32_t *new_start, const uint32_t *deg, uint32_t n,
                             hipStream_t s);
 int dfs_max_blocks_per_cu(uint32_t waves_per_block, uint32_t max_query_vertices);
+
+// Kernel-node parameters of one DFS round for a HIP graph (ctl reset, two seed
+// lists, the DFS), the same launches launch_ctl_reset / launch_seed /
+// launch_dfs make. The node's kernelParams point into `st`, which must stay
+// valid until the node is added or updated (the values are copied then). A
+// seed list with no records still gets one block, which does nothing.
+struct RoundNodeArgs {
+  DfsCtl *ctl;
+  ApplyCtl *no_actl;
+  unsigned long long *no_tc;
+  unsigned long long *no_counts;
+  uint32_t no_ncounts;
+  SeedArgs seed[2];
+  DfsArgs dfs;
+  uint32_t chunk_levels;
+  void *reset_params[5];
+  void *seed_params[2][1];
+  void *dfs_params[2];
+};
+void round_node_params(RoundNodeArgs &st, uint32_t blocks, uint32_t waves_per_block,
+                       uint32_t max_query_vertices, hipKernelNodeParams *reset, hipKernelNodeParams *seed0,
+                       hipKernelNodeParams *seed1, hipKernelNodeParams *dfs);
 uint32_t dfs_lds_bytes_per_wave(uint32_t max_query_vertices);
 
 }  // namespace dsm

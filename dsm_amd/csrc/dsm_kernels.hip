@@ -1419,6 +1419,60 @@ hipError_t launch_dfs(const DfsArgs &a, uint32_t blocks, uint32_t waves_per_bloc
   return hipGetLastError();
 }
 
+void round_node_params(RoundNodeArgs &st, uint32_t blocks, uint32_t waves_per_block,
+                       uint32_t max_query_vertices, hipKernelNodeParams *reset, hipKernelNodeParams *seed0,
+                       hipKernelNodeParams *seed1, hipKernelNodeParams *dfs) {
+  st.no_actl = nullptr;
+  st.no_tc = nullptr;
+  st.no_counts = nullptr;
+  st.no_ncounts = 0;
+  st.reset_params[0] = &st.ctl;
+  st.reset_params[1] = &st.no_actl;
+  st.reset_params[2] = &st.no_tc;
+  st.reset_params[3] = &st.no_counts;
+  st.reset_params[4] = &st.no_ncounts;
+  *reset = hipKernelNodeParams{};
+  reset->func = reinterpret_cast<void *>(&k_ctl_reset);
+  reset->gridDim = dim3(1);
+  reset->blockDim = dim3(256);
+  reset->sharedMemBytes = 0;
+  reset->kernelParams = st.reset_params;
+  reset->extra = nullptr;
+  hipKernelNodeParams *sp[2] = {seed0, seed1};
+  for (int i = 0; i < 2; i++) {
+    st.seed_params[i][0] = &st.seed[i];
+    const uint32_t b = st.seed[i].nrec ? (st.seed[i].nrec + 255) / 256 : 1u;
+    *sp[i] = hipKernelNodeParams{};
+    sp[i]->func = reinterpret_cast<void *>(&k_seed);
+    sp[i]->gridDim = dim3(b < 4096 ? b : 4096);
+    sp[i]->blockDim = dim3(256);
+    sp[i]->sharedMemBytes = 0;
+    sp[i]->kernelParams = st.seed_params[i];
+    sp[i]->extra = nullptr;
+  }
+  st.chunk_levels = max_query_vertices > 3 ? max_query_vertices - 3 : 1;
+  st.dfs_params[0] = &st.dfs;
+  st.dfs_params[1] = &st.chunk_levels;
+  const size_t per = wave_lds_bytes(st.chunk_levels);
+  uint32_t wpb = st.dfs.tuples ? 4u : waves_per_block;
+  void *fn;
+  if (st.dfs.tuples) fn = reinterpret_cast<void *>(&k_dfs<4, true>);
+  else if (wpb == 1) fn = reinterpret_cast<void *>(&k_dfs<1, false>);
+  else if (wpb == 2) fn = reinterpret_cast<void *>(&k_dfs<2, false>);
+  else if (wpb == 8) fn = reinterpret_cast<void *>(&k_dfs<8, false>);
+  else {
+    wpb = 4;
+    fn = reinterpret_cast<void *>(&k_dfs<4, false>);
+  }
+  *dfs = hipKernelNodeParams{};
+  dfs->func = fn;
+  dfs->gridDim = dim3(blocks);
+  dfs->blockDim = dim3(wpb * 64);
+  dfs->sharedMemBytes = (unsigned int)(wpb * per);
+  dfs->kernelParams = st.dfs_params;
+  dfs->extra = nullptr;
+}
+
 uint32_t dfs_lds_bytes_per_wave(uint32_t max_query_vertices) {
   return wave_lds_bytes(max_query_vertices > 3 ? max_query_vertices - 3 : 1);
 }

@@ -66,7 +66,7 @@ class Options(C.Structure):
                 ("tuple_capacity", C.c_uint64), ("tuples_in_host_memory", C.c_int),
                 ("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("donate_min", C.c_uint32), ("queue_capacity", C.c_uint32), ("watchdog_ms", C.c_uint32),
-                ("lane_work", C.c_uint32), ("leaf_sets", C.c_int)]
+                ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("graph_rounds", C.c_int)]
 
 
 class TuneReport(C.Structure):
@@ -243,7 +243,7 @@ class Engine:
 
     def __init__(self, graph, queries, device=0, relabel=True, count_negatives=True, tuple_capacity=0,
                  tuples_in_host_memory=False, waves_per_block=4, blocks_per_cu=0, donate_min=0,
-                 queue_capacity=0, watchdog_ms=0, lane_work=0, leaf_sets=True):
+                 queue_capacity=0, watchdog_ms=0, lane_work=0, leaf_sets=True, graph_rounds=True):
         L = lib()
         n, lab, eu, ev = graph
         self._keep = []
@@ -269,6 +269,7 @@ class Engine:
         opt.watchdog_ms = watchdog_ms
         opt.lane_work = lane_work
         opt.leaf_sets = int(bool(leaf_sets))
+        opt.graph_rounds = int(bool(graph_rounds))
         h = C.c_void_p()
         rc = L.dsm_create(C.byref(gd), qds, len(queries), C.byref(opt), C.byref(h))
         self._keep = None

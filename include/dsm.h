@@ -84,6 +84,9 @@ typedef struct {
   int leaf_sets;             /* 1: one level above the leaf, candidates count their
                                 leaf images against a leaf set shared by the parent;
                                 0: one descent per candidate */
+  int graph_rounds;          /* 1: each DFS round (control reset, seeding, DFS, status
+                                copy) is one HIP graph launch, built once and updated
+                                per round; 0: separate launches */
 } dsm_options;
 
 /* Per-batch statistics of the last dsm_apply_batch + dsm_match_batch. */

@@ -381,3 +381,19 @@ def test_leaf_level_strategies_agree(leaf_sets, lane_work):
     present = set(zip(eu.tolist(), ev.tolist()))
     batches = [conflict_free_batch(n, 60, present, rng) for _ in range(2)]
     compare_stream(n, labels, eu, ev, queries, batches, relabel=True, leaf_sets=leaf_sets, lane_work=lane_work)
+
+
+@pytest.mark.parametrize("tuples", [False, True])
+def test_graph_and_eager_rounds_agree(tuples):
+    """The DFS round as a HIP graph (built once, node arguments updated per
+    round, replayed with eager apply work in between) against separate
+    launches and the oracle, batch by batch, tuples included."""
+    say("case: graph rounds vs eager rounds, tuples=%s" % tuples)
+    rng = np.random.default_rng(1313)
+    n, m, nl = 300, 3000, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple("triangle", [0, 1, 0]), qtuple("house", [0, 1, 0, 1, 0]), qtuple("path6", [0, 1, 0, 1, 0, 1])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batches = [conflict_free_batch(n, int(k), present, rng) for k in (1, 40, 3, 120, 0, 60)]
+    for graph_rounds in (True, False):
+        compare_stream(n, labels, eu, ev, queries, batches, relabel=True, tuples=tuples, graph_rounds=graph_rounds)
