@@ -400,10 +400,68 @@ __global__ __launch_bounds__(256) void k_filter_heavy(DevGraph g, FilterTables f
 
 // ---------------------------------------------------- label-sorted copies
 
-// One wave per vertex of degree > kLabelSorted: counts its neighbours per
-// label bucket (LDS atomics), writes the bucket offsets, then scatters the
-// id-sorted list into lpool bucket by bucket, keeping id order inside each
-// bucket (per chunk, one ballot per distinct bucket present).
+// Lanes of the wave whose bucket equals this lane's (among valid lanes): one
+// ballot per bit of the bucket index, so a chunk with many distinct buckets
+// costs the same as one with a single bucket.
+__device__ __forceinline__ uint64_t bucket_peers(uint32_t b, bool valid, uint32_t bits) {
+  uint64_t m = ballot(valid);
+  for (uint32_t i = 0; i < bits; i++) {
+    const bool set = (b >> i) & 1;
+    const uint64_t bi = ballot(valid && set);
+    m &= set ? bi : ~bi;
+  }
+  return m;
+}
+
+__device__ __forceinline__ uint32_t bucket_of(const DevGraph &g, const uint8_t *s_lmap, uint32_t x) {
+  const uint32_t b = s_lmap[g.lab[x]];
+  return b == 0xFF ? g.nbuckets - 1 : b;
+}
+
+// Adds the bucket histogram of A[lo, hi) to cnt (per-wave LDS counters; the
+// lowest lane of each peer group adds the group's size).
+__device__ __forceinline__ void bucket_count(const DevGraph &g, const uint8_t *s_lmap, const uint32_t *A,
+                                             uint32_t lo, uint32_t hi, uint32_t bits, uint32_t *cnt) {
+  const uint32_t lane = lane_id();
+  for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
+    const uint32_t j = c0 + lane;
+    const bool valid = j < hi;
+    const uint32_t b = valid ? bucket_of(g, s_lmap, A[j]) : 0u;
+    const uint64_t pm = bucket_peers(b, valid, bits);
+    if (valid && mbcnt(pm) == 0) cnt[b] += (uint32_t)__popcll(pm);
+    wave_sync();
+  }
+}
+
+// Scatters A[lo, hi) into out by bucket, cnt[b] holding the next position of
+// bucket b; id order is kept inside each bucket.
+__device__ __forceinline__ void bucket_scatter(const DevGraph &g, const uint8_t *s_lmap, const uint32_t *A,
+                                               uint32_t lo, uint32_t hi, uint32_t bits, uint32_t *cnt,
+                                               uint32_t *out) {
+  const uint32_t lane = lane_id();
+  for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
+    const uint32_t j = c0 + lane;
+    const bool valid = j < hi;
+    const uint32_t x = valid ? A[j] : 0u;
+    const uint32_t b = valid ? bucket_of(g, s_lmap, x) : 0u;
+    const uint64_t pm = bucket_peers(b, valid, bits);
+    const uint32_t rank = mbcnt(pm);
+    const uint32_t base = valid ? cnt[b] : 0u;
+    wave_sync();
+    if (valid) {
+      out[base + rank] = x;
+      if (rank == 0) cnt[b] = base + (uint32_t)__popcll(pm);
+    }
+    wave_sync();
+  }
+}
+
+constexpr uint32_t kLbuildBlock = 4096;  // lists longer than this are sorted by a whole block
+constexpr uint32_t kLbuildWaves = 16;    // waves of that block
+
+// One wave per vertex of degree in (kLabelSorted, kLbuildBlock]: bucket
+// histogram, offsets, then a stable scatter of the id-sorted list into lpool
+// bucket by bucket. Longer lists are left to k_lbuild_big.
 __global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap, const uint32_t *list,
                                                 const uint32_t *count) {
   __shared__ uint8_t s_lmap[256];
@@ -412,23 +470,18 @@ __global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap,
   __syncthreads();
   const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
   const uint32_t nb = g.nbuckets;
+  const uint32_t bits = 32u - (uint32_t)__builtin_clz(nb > 1 ? nb - 1 : 1u);
   const uint32_t total = list ? (*count < g.n ? *count : g.n) : g.n;
   uint32_t *cnt = s_cnt[wv];
   for (uint32_t t = blockIdx.x * 4 + wv; t < total; t += gridDim.x * 4) {
     const uint32_t v = list ? list[t] : t;
     const uint32_t d = g.deg[v];
-    if (d <= kLabelSorted) continue;
+    if (d <= kLabelSorted || d > kLbuildBlock) continue;
     const uint32_t *A = g.pool + g.start[v];
-    uint32_t *out = g.lpool + g.start[v];
     uint32_t *row = g.loff + (size_t)v * (nb + 1);
     if (lane < nb) cnt[lane] = 0;
     wave_sync();
-    for (uint32_t j = lane; j < d; j += 64) {
-      uint32_t b = s_lmap[g.lab[A[j]]];
-      if (b == 0xFF) b = nb - 1;
-      atomicAdd(&cnt[b], 1u);
-    }
-    wave_sync();
+    bucket_count(g, s_lmap, A, 0, d, bits, cnt);
     if (lane == 0) {
       uint32_t run = 0;
       for (uint32_t k = 0; k < nb; k++) {
@@ -440,24 +493,68 @@ __global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap,
       row[nb] = run;
     }
     wave_sync();
-    for (uint32_t c0 = 0; c0 < d; c0 += 64) {
-      const uint32_t j = c0 + lane;
-      const bool valid = j < d;
-      const uint32_t x = valid ? A[j] : 0u;
-      uint32_t b = valid ? s_lmap[g.lab[x]] : 0u;
-      if (b == 0xFF) b = nb - 1;
-      uint64_t rest = ballot(valid);
-      while (rest) {
-        const uint32_t L = __shfl(b, (int)__builtin_ctzll(rest));
-        const uint64_t m = ballot(valid && b == L);
-        const uint32_t base = cnt[L];
-        if (valid && b == L) out[base + mbcnt(m)] = x;
-        wave_sync();
-        if (lane == 0) cnt[L] = base + (uint32_t)__popcll(m);
-        wave_sync();
-        rest &= ~m;
+    bucket_scatter(g, s_lmap, A, 0, d, bits, cnt, g.lpool + g.start[v]);
+  }
+}
+
+// One block of kLbuildWaves waves per vertex of degree > kLbuildBlock: each
+// wave takes a contiguous segment of the list; per-wave histograms give every
+// wave its starting position inside every bucket, so the block-wide scatter
+// stays stable (id order inside each bucket). Control flow around the block
+// barriers depends only on the block's vertex, so it is uniform.
+__global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_big(DevGraph g, const uint8_t *lmap,
+                                                                  const uint32_t *list, const uint32_t *count) {
+  __shared__ uint8_t s_lmap[256];
+  __shared__ uint32_t s_cnt[kLbuildWaves][kMaxRel + 2];
+  __shared__ uint32_t s_base[kMaxRel + 2];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = lmap[i];
+  __syncthreads();
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t nb = g.nbuckets;
+  const uint32_t bits = 32u - (uint32_t)__builtin_clz(nb > 1 ? nb - 1 : 1u);
+  const uint32_t total = list ? (*count < g.n ? *count : g.n) : g.n;
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const uint32_t v = list ? list[t] : t;
+    const uint32_t d = g.deg[v];
+    if (d <= kLbuildBlock) continue;
+    const uint32_t *A = g.pool + g.start[v];
+    uint32_t *row = g.loff + (size_t)v * (nb + 1);
+    const uint32_t seg = (d + kLbuildWaves - 1) / kLbuildWaves;
+    const uint32_t lo = wv * seg < d ? wv * seg : d;
+    const uint32_t hi = lo + seg < d ? lo + seg : d;
+    uint32_t *cnt = s_cnt[wv];
+    if (lane < nb) cnt[lane] = 0;
+    wave_sync();
+    bucket_count(g, s_lmap, A, lo, hi, bits, cnt);
+    __syncthreads();
+    if (threadIdx.x < nb) {
+      uint32_t tot = 0;
+      for (uint32_t w = 0; w < kLbuildWaves; w++) tot += s_cnt[w][threadIdx.x];
+      s_base[threadIdx.x] = tot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (uint32_t k = 0; k < nb; k++) {
+        const uint32_t c = s_base[k];
+        s_base[k] = run;
+        row[k] = run;
+        run += c;
+      }
+      row[nb] = run;
+    }
+    __syncthreads();
+    if (threadIdx.x < nb) {
+      uint32_t run = s_base[threadIdx.x];
+      for (uint32_t w = 0; w < kLbuildWaves; w++) {
+        const uint32_t c = s_cnt[w][threadIdx.x];
+        s_cnt[w][threadIdx.x] = run;
+        run += c;
       }
     }
+    __syncthreads();
+    bucket_scatter(g, s_lmap, A, lo, hi, bits, cnt, g.lpool + g.start[v]);
+    __syncthreads();
   }
 }
 
@@ -1392,6 +1489,9 @@ hipError_t launch_lbuild(const DevGraph &g, const uint8_t *lmap, const uint32_t 
                          hipStream_t s) {
   const uint32_t blocks = list ? 2048u : ((g.n + 3) / 4 < 8192 ? (g.n + 3) / 4 : 8192u);
   hipLaunchKernelGGL(k_lbuild, dim3(blocks), dim3(256), 0, s, g, lmap, list, count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lbuild_big, dim3(list ? 256u : 1024u), dim3(kLbuildWaves * 64), 0, s, g, lmap, list, count);
   return hipGetLastError();
 }
 

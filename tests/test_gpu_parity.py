@@ -397,3 +397,34 @@ def test_graph_and_eager_rounds_agree(tuples):
     batches = [conflict_free_batch(n, int(k), present, rng) for k in (1, 40, 3, 120, 0, 60)]
     for graph_rounds in (True, False):
         compare_stream(n, labels, eu, ev, queries, batches, relabel=True, tuples=tuples, graph_rounds=graph_rounds)
+
+
+def test_label_sorted_copy_of_big_hub():
+    """A hub of degree > 4,096 (its label-sorted copy is built by a whole
+    block) and relabels of its neighbours every batch (the copy is rebuilt
+    for the dirty set): counts, graph and bitmaps against the oracle."""
+    say("case: hub of degree 9,000 with relabelled neighbours")
+    rng = np.random.default_rng(1414)
+    n, nl = 10000, 4
+    labels = rng.integers(0, nl, n).astype(np.uint32)
+    hub_nbrs = rng.choice(np.arange(1, n), 9000, replace=False)
+    edges = set((0, int(v)) for v in hub_nbrs)
+    while len(edges) < 9000 + 20000:
+        a, b = map(int, rng.integers(1, n, 2))
+        if a != b:
+            edges.add((min(a, b), max(a, b)))
+    e = np.array(sorted(edges), np.uint32)
+    eu, ev = e[:, 0].copy(), e[:, 1].copy()
+    queries = [qtuple("triangle", [int(labels[0]), 1, 2]), qtuple("square", [int(labels[0]), 0, 3, 1]),
+               qtuple("house", [int(labels[0]), 1, 0, 2, 3])]
+    present = set(edges)
+    batches = []
+    for _ in range(3):
+        add, src, dst, lab = conflict_free_batch(n, 300, present, rng)
+        lab = rng.integers(0, nl, len(src)).astype(np.uint32)   # relabels both endpoints of inserts
+        batches.append((add, src, dst, lab))
+    # inserts at the hub so that its list and its neighbours' labels change
+    k = 50
+    nb = rng.choice(np.setdiff1d(np.arange(1, n), hub_nbrs), k, replace=False).astype(np.uint32)
+    batches.append((np.ones(k, np.uint8), np.zeros(k, np.uint32), nb, rng.integers(0, nl, k).astype(np.uint32)))
+    compare_stream(n, labels, eu, ev, queries, batches, relabel=True, check_graph=True)
