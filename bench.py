@@ -48,9 +48,15 @@ def make_workload(name):
         return W.rmat_q64()
     if name == "rmat1m":
         return W.rmat()
+    if name == "rmat16m":
+        return W.rmat16m()
     if name == "small":
         return W.small()
     raise SystemExit("unknown workload %s" % name)
+
+
+GRAPH_TEXT = {"rmat1m_q64": "R-MAT 1M V / 10M E, 16 Zipf(1.0) labels", "rmat1m": "R-MAT 1M V / 10M E, 16 Zipf(1.0) labels",
+              "rmat16m": "R-MAT 16M V / 256M E, 32 Zipf(0.8) labels", "small": "uniform 1k V / 8k E, 4 labels"}
 
 
 def qtuples(w):
@@ -310,6 +316,7 @@ def main():
                  "blocks_per_cu": args.blocks_per_cu, "lane_work": args.lane_work,
                  "leaf_sets": bool(args.leaf_sets)}
 
+    batch = int(w.bstart[1] - w.bstart[0])
     bat = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune)
     total_s = sum(bat["lat"])
     value = bat["records"] / total_s
@@ -344,10 +351,10 @@ def main():
            "value": round(value, 1), "unit": "edge updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-           "config": {"workload": "%s: R-MAT 1M V / 10M E, 16 Zipf labels, 4,096-record batches, %d standing "
-                                  "queries, batched (one persistent launch per graph state)"
-                                  % (w.name, len(w.queries)),
-                      "batch": 4096, "queries": len(w.queries), "mode": "batched",
+           "config": {"workload": "%s: %s, %d-record batches, %d standing queries, batched (one persistent "
+                                  "launch per graph state)" % (w.name, GRAPH_TEXT.get(w.name, ""), batch,
+                                                                len(w.queries)),
+                      "batch": batch, "queries": len(w.queries), "mode": "batched",
                       "parallelism": "single GPU", "tuned": bat["tuned"]},
            "p50_batch_ms": round(pct(bat["lat"], 50) * 1e3, 3), "p99_batch_ms": round(pct(bat["lat"], 99) * 1e3, 3),
            "matches_per_s": round((bat["pos"] + bat["neg"]) / total_s, 1),

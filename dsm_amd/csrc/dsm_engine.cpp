@@ -220,7 +220,9 @@ namespace {
 
 // ------------------------------------------------------------ query set-up
 
-int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq) {
+// label_freq[l]: data vertices of label l (l < 255), used only to order the
+// matching: it never changes a count.
+int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq, const std::vector<uint64_t> &label_freq) {
   if (nq == 0 || nq > DSM_MAX_QUERIES) return e->fail_msg(DSM_EINVAL, "need 1..%d queries", DSM_MAX_QUERIES);
   e->queries.resize(nq);
   for (uint32_t q = 0; q < nq; q++) {
@@ -324,14 +326,18 @@ int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq) {
         order[1] = v;
         uint32_t placed = (1u << u) | (1u << v);
         for (uint32_t k = 2; k < Q.n; k++) {
-          // most earlier neighbours, then highest query degree, then lowest id
+          // most earlier neighbours, then the rarest label in the data graph,
+          // then highest query degree, then lowest id
           int best = -1, bb = -1, bd = -1;
+          uint64_t bf = ~0ull;
           for (uint32_t w = 0; w < Q.n; w++) {
             if (placed >> w & 1) continue;
-            int b = __builtin_popcount(Q.adj[w] & placed), dg = __builtin_popcount(Q.adj[w]);
-            if (b > bb || (b == bb && dg > bd)) {
+            const int b = __builtin_popcount(Q.adj[w] & placed), dg = __builtin_popcount(Q.adj[w]);
+            const uint64_t f = Q.lab[w] < label_freq.size() ? label_freq[Q.lab[w]] : 0;
+            if (b > bb || (b == bb && (f < bf || (f == bf && dg > bd)))) {
               best = (int)w;
               bb = b;
+              bf = f;
               bd = dg;
             }
           }
@@ -886,7 +892,10 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   if (e->wall_khz <= 0) e->wall_khz = 100000;
   if (e->opt.watchdog_ms) e->watchdog_ms = e->opt.watchdog_ms;
   for (auto &x : e->ev) HIPCHK(hipEventCreate(&x));
-  if ((rc = build_queries(e, queries, num_queries))) return rc;
+  std::vector<uint64_t> label_freq(256, 0);
+  for (uint32_t v = 0; graph->labels && v < graph->num_vertices; v++)
+    if (graph->labels[v] < 255) label_freq[graph->labels[v]]++;
+  if ((rc = build_queries(e, queries, num_queries, label_freq))) return rc;
   if ((rc = upload_graph(e, graph))) return rc;
   if ((rc = upload_tables(e))) return rc;
   uint32_t wpb = e->opt.waves_per_block;

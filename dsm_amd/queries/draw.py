@@ -6,9 +6,12 @@
                    chorded cycles; shapes from workloads.q64_specs, shrunk by
                    a vertex at a time when no draw fits), pilot budget 2e7
                    nodes each
+  (both over the records of the first 12 batches of the R-MAT 1M / 10M stream)
+  rmat16m.json     4-clique, 5-cycle with two chords, 7-vertex tree on the
+                   R-MAT 16M / 256M graph, pilot budget 2e8 nodes each over
+                   the records of the first 2 batches (65,536 records)
 
-Both over the records of the first 12 batches of the R-MAT 1M / 10M stream
-(see workloads.draw_queries for the rule). Run from the repository root:
+See workloads.draw_queries for the rule. Run from the repository root:
     python -m dsm_amd.queries.draw [rmat1m|rmat1m_q64]
 """
 import json
@@ -19,8 +22,9 @@ import time
 from dsm_amd import workloads as W
 
 SETS = {
-    "rmat1m": {"specs": [("cycle5_chord", 6)], "budget": 50_000_000},
-    "rmat1m_q64": {"specs": W.q64_specs(), "budget": 20_000_000},
+    "rmat1m": {"make": W.rmat, "specs": [("cycle5_chord", 6)], "budget": 50_000_000, "window": 12, "nl": 16},
+    "rmat1m_q64": {"make": W.rmat, "specs": W.q64_specs(), "budget": 20_000_000, "window": 12, "nl": 16},
+    "rmat16m": {"make": W.rmat16m, "specs": W.RMAT16M_SPECS, "budget": 200_000_000, "window": 2, "nl": 32},
 }
 
 
@@ -28,13 +32,14 @@ def main(names):
     for name in names:
         cfg = SETS[name]
         t = time.time()
-        w = W.rmat(name=name, queries=[])
-        qs = W.draw_queries(w, cfg["specs"], seed=2, nl=16, budget=cfg["budget"],
+        w = cfg["make"](name=name, queries=[])
+        print("%s: workload built in %.1f s" % (name, time.time() - t), flush=True)
+        qs = W.draw_queries(w, cfg["specs"], seed=2, nl=cfg["nl"], budget=cfg["budget"], window=cfg["window"],
                             log=lambda m: print(m, flush=True))
-        out = {"workload": name, "rule": "randomized walk; redrawn while the pilot over the first 12 batches "
+        out = {"workload": name, "rule": "randomized walk; redrawn while the pilot over the first %d batches "
                                           "exceeds the budget; rarer labels every 8 redraws; one vertex smaller "
-                                          "after 160 draws",
-               "budget_nodes": cfg["budget"], "window_batches": 12,
+                                          "after 160 draws" % cfg["window"],
+               "budget_nodes": cfg["budget"], "window_batches": cfg["window"],
                "queries": [{"shape": q.shape, "labels": q.labels.tolist(), "edges": q.edges,
                             "pilot_nodes": q.cost, "draw": q.draw} for q in qs]}
         with open(os.path.join(W.QUERY_DIR, name + ".json"), "w") as f:

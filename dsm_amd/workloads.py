@@ -16,6 +16,10 @@ Shapes follow the workload definitions of the reference's benchmark plan:
           batches of 4,096; one 6-vertex query (5-cycle plus chord).
   rmat1m_q64  the same graph and stream with 64 standing queries of 4-12
           vertices (trees, cycles, 4-cliques, chorded cycles).
+  rmat16m R-MAT (.57,.19,.19,.05), 16,000,000 V, 256,000,000 undirected
+          edges, 32 Zipf(0.8) labels; the same stream construction in batches
+          of 32,768; dense queries: 4-clique, 5-cycle with two chords,
+          7-vertex tree.
 
 Label field of an insert record: the ORIGINAL label of its source vertex. Under
 the reference update rule (cpuGraph.cpp:564) an insert relabels both endpoints
@@ -368,3 +372,14 @@ def rmat_q64(**kw):
     """The same graph and stream with the 64 committed standing queries."""
     kw.setdefault("name", "rmat1m_q64")
     return rmat(**kw)
+
+
+RMAT16M_SPECS = [("clique4", 4), ("cycle5_2chords", 5), ("tree", 7)]
+
+
+def rmat16m(**kw):
+    """R-MAT 16,000,000 V / 256,000,000 undirected edges, 32 Zipf(0.8) labels,
+    batches of 32,768, with the committed dense queries (4-clique, 5-cycle with
+    two chords, 7-vertex tree)."""
+    kw.setdefault("name", "rmat16m")
+    return rmat(n=16_000_000, m=256_000_000, nl=32, zipf=0.8, batch=32768, seed=5, **kw)
