@@ -24,7 +24,10 @@ from dsm_amd import workloads as W
 SETS = {
     "rmat1m": {"make": W.rmat, "specs": [("cycle5_chord", 6)], "budget": 50_000_000, "window": 12, "nl": 16},
     "rmat1m_q64": {"make": W.rmat, "specs": W.q64_specs(), "budget": 20_000_000, "window": 12, "nl": 16},
-    "rmat16m": {"make": W.rmat16m, "specs": W.RMAT16M_SPECS, "budget": 200_000_000, "window": 2, "nl": 32},
+    # dense shapes sit around hubs: the walk may start and step anywhere, and a
+    # shape never falls back to a cycle or tree
+    "rmat16m": {"make": W.rmat16m, "specs": W.RMAT16M_SPECS, "budget": 200_000_000, "window": 2, "nl": 32,
+                "walk": (1 << 30, 1 << 30, 200000, 400), "fallback": False},
 }
 
 
@@ -35,7 +38,8 @@ def main(names):
         w = cfg["make"](name=name, queries=[])
         print("%s: workload built in %.1f s" % (name, time.time() - t), flush=True)
         qs = W.draw_queries(w, cfg["specs"], seed=2, nl=cfg["nl"], budget=cfg["budget"], window=cfg["window"],
-                            log=lambda m: print(m, flush=True))
+                            log=lambda m: print(m, flush=True), walk=cfg.get("walk"),
+                            fallback=cfg.get("fallback", True))
         out = {"workload": name, "rule": "randomized walk; redrawn while the pilot over the first %d batches "
                                           "exceeds the budget; rarer labels every 8 redraws; one vertex smaller "
                                           "after 160 draws" % cfg["window"],

@@ -268,13 +268,15 @@ MIN_SIZE = {"tree": 3, "cycle": 3, "chord_cycle": 4, "clique4": 4, "cycle5_chord
             "cycle5_2chords": 5}
 
 
-def _pick_query(n, off, nbr, labels, nl, kind, k, seed0, rs, rd, budget, max_tries):
+def _pick_query(n, off, nbr, labels, nl, kind, k, seed0, rs, rd, budget, max_tries, walk=(256, 1024, 5000, 50)):
     """Draws until the pilot fits the budget; every 8 redraws the walk moves
-    to rarer labels; after max_tries draws the shape shrinks by one vertex."""
+    to rarer labels; after max_tries draws the shape shrinks by one vertex.
+    walk = (max start degree, max walk degree, walk budget, attempts)."""
     while k >= MIN_SIZE.get(kind, 3):
         for t in range(max_tries):
-            q = extract_query(n, off, nbr, labels, kind, k, seed0 + k * 4096 + t, max_start_deg=256,
-                              max_walk_deg=1024, budget=5000, attempts=50, min_label=min(nl - 1, t // 8))
+            q = extract_query(n, off, nbr, labels, kind, k, seed0 + k * 4096 + t, max_start_deg=walk[0],
+                              max_walk_deg=walk[1], budget=walk[2], attempts=walk[3],
+                              min_label=min(nl - 1, t // 8))
             if q is None:
                 continue
             q.cost = pilot_nodes(n, off, nbr, labels, q, rs, rd, budget)
@@ -285,7 +287,8 @@ def _pick_query(n, off, nbr, labels, nl, kind, k, seed0, rs, rd, budget, max_tri
     raise RuntimeError("no query of shape %s within the budget" % kind)
 
 
-def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=160, threads=8, log=None):
+def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=160, threads=8, log=None, walk=None,
+                 fallback=True):
     """Draws one query per (shape, size) spec by randomized walk on the initial
     graph of w. A draw whose backtracking tree over the records of the first
     `window` batches (pilot on the initial graph, pilot_nodes) has more than
@@ -307,10 +310,10 @@ def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=160, threads=8
         q = None
         # a shape with no draw within the budget falls back to a cycle, then a
         # tree, of the same size (recorded in q.shape)
-        for alt in (kind, "cycle", "tree"):
+        for alt in ((kind, "cycle", "tree") if fallback else (kind,)):
             try:
                 q = _pick_query(w.n, off, nbr, w.labels, nl, alt, k, seed * 1000 + i * 1024 + 30, rs, rd,
-                                budget, max_tries)
+                                budget, max_tries, **({"walk": walk} if walk else {}))
                 break
             except RuntimeError:
                 continue
