@@ -82,7 +82,8 @@ def run_batched(w, warmup, steps, engine_kw, autotune=True):
         tuned = eng.autotune(*tune_sample(w))
         log("autotune %.2f s: %s" % (time.perf_counter() - t0, tuned))
     lat, recs = [], 0
-    agg = {"ms_dfs": 0.0, "launches": 0, "entries": 0, "loads": 0, "merge_bytes": 0, "ms_apply": 0.0,
+    agg = {"ms_dfs": 0.0, "launches": 0, "entries": 0, "loads": 0, "merge_bytes": 0, "busy_ms": 0.0, "waves": 0,
+           "imbalance": 0.0, "ms_apply": 0.0,
            "ms_filter": 0.0,
            "ms_seed": 0.0, "seeds": 0, "donated": 0}
     pos_tot = neg_tot = 0
@@ -107,6 +108,9 @@ def run_batched(w, warmup, steps, engine_kw, autotune=True):
             agg["entries"] += st["dfs_entries"]
             agg["loads"] += st["dfs_search_loads"]
             agg["merge_bytes"] += st["dfs_merge_bytes"]
+            agg["busy_ms"] += st["dfs_busy_ms"]
+            agg["waves"] += st["dfs_waves"]
+            agg["imbalance"] += st["dfs_block_imbalance"]
             agg["ms_apply"] += st["ms_apply"]
             agg["ms_filter"] += st["ms_filter"]
             agg["ms_seed"] += st["ms_seed"]
@@ -298,6 +302,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--lane-work", type=int, default=0)
     ap.add_argument("--leaf-sets", type=int, default=1)
+    ap.add_argument("--donate-every", type=int, default=0)
+    ap.add_argument("--donate-min-n2", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run one D2D copy of this many bytes")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -314,7 +320,8 @@ def main():
         % (w.name, time.perf_counter() - t, w.n, len(w.eu), w.num_batches, len(w.queries)))
     engine_kw = {"waves_per_block": args.waves_per_block, "donate_min": args.donate_min,
                  "blocks_per_cu": args.blocks_per_cu, "lane_work": args.lane_work,
-                 "leaf_sets": bool(args.leaf_sets)}
+                 "leaf_sets": bool(args.leaf_sets), "donate_every": args.donate_every,
+                 "donate_min_n2": args.donate_min_n2}
 
     batch = int(w.bstart[1] - w.bstart[0])
     bat = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune)
@@ -360,7 +367,13 @@ def main():
            "matches_per_s": round((bat["pos"] + bat["neg"]) / total_s, 1),
            "stages_ms_per_batch": {k: round(agg[k] / len(bat["lat"]), 3) for k in ("ms_apply", "ms_filter",
                                                                                "ms_seed", "ms_dfs")},
-           "roofline": roofline}
+           "roofline": roofline,
+           # persistent-grid balance of the DFS: share of the waves' time spent on
+           # tasks (the rest is waiting for donated work), and the busiest block's
+           # task time over the mean block's, per launch
+           "dfs_balance": {"wave_utilization": round(agg["busy_ms"] / max(agg["waves"] / nl * agg["ms_dfs"], 1e-9), 4),
+                           "block_imbalance": round(agg["imbalance"] / nl, 4),
+                           "waves_per_launch": int(agg["waves"] / nl)}}
 
     if not args.no_plain:
         ps = min(args.plain_steps, args.steps)

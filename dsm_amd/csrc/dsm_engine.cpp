@@ -134,9 +134,13 @@ struct dsm_engine {
   bool tuples_host = false;
   unsigned long long tuple_cap = 0;
   uint32_t dfs_blocks = 0, dfs_wpb = 4, max_qn = 2;
-  uint32_t donate_min = 256, lane_work = 16;
+  uint32_t donate_min = 256, lane_work = 16, donate_every = 32, donate_min_n2 = 64;
   bool leaf_sets = true;
   bool graph_rounds = true;
+  // per-wave task time of the last DFS launch (balance statistics)
+  DevBuf<unsigned long long> wave_busy;
+  PinnedBuf<unsigned long long> h_wave_busy;
+  uint32_t wave_slots = 0, last_blocks = 0, last_wpb = 0;
   // the DFS round as a HIP graph (graph_round)
   hipGraph_t rg_graph = nullptr;
   hipGraphExec_t rg_exec = nullptr;
@@ -548,6 +552,8 @@ DfsArgs dfs_args(dsm_engine *e) {
   da.epoch = ++e->epoch;
   da.donate_min = e->donate_min;
   da.lane_work = e->lane_work;
+  da.donate_every = e->donate_every;
+  da.donate_min_n2 = e->donate_min_n2;
   da.leaf_sets = e->leaf_sets ? 1u : 0u;
   da.nq = (uint32_t)e->queries.size();
   da.watchdog = (unsigned long long)e->wall_khz * e->watchdog_ms;
@@ -556,6 +562,8 @@ DfsArgs dfs_args(dsm_engine *e) {
   da.tuples = e->tuples;
   da.tuple_cap = e->tuple_cap;
   da.tuple_count = e->tuple_count.p;
+  da.wave_busy = e->wave_busy.p;
+  da.wave_slots = e->wave_slots;
   return da;
 }
 
@@ -564,7 +572,10 @@ DfsArgs dfs_args(dsm_engine *e) {
 uint32_t dfs_grid(dsm_engine *e, uint32_t nrec) {
   const uint64_t want =
       ((uint64_t)nrec * std::max<uint32_t>(e->max_plans_per_key, 1) * 8 + e->dfs_wpb - 1) / e->dfs_wpb;
-  return (uint32_t)std::min<uint64_t>(e->dfs_blocks, std::max<uint64_t>(want, (uint64_t)e->num_cu));
+  const uint32_t b = (uint32_t)std::min<uint64_t>(e->dfs_blocks, std::max<uint64_t>(want, (uint64_t)e->num_cu));
+  e->last_blocks = b;
+  e->last_wpb = e->tuples ? 4u : e->dfs_wpb;
+  return b;
 }
 
 void drop_round_graph(dsm_engine *e) {
@@ -595,9 +606,11 @@ int graph_round(dsm_engine *e, const RoundPart *parts, int nparts, uint32_t nrec
     HIPCHK(hipGraphAddKernelNode(&e->rg_seed0, e->rg_graph, &e->rg_reset, 1, &p0));
     HIPCHK(hipGraphAddKernelNode(&e->rg_seed1, e->rg_graph, &e->rg_seed0, 1, &p1));
     HIPCHK(hipGraphAddKernelNode(&e->rg_dfs, e->rg_graph, &e->rg_seed1, 1, &pd));
-    hipGraphNode_t cp;
+    hipGraphNode_t cp, cw;
     HIPCHK(hipGraphAddMemcpyNode1D(&cp, e->rg_graph, &e->rg_dfs, 1, e->h_ctl.p, e->ctl.p, sizeof(DfsCtl),
                                    hipMemcpyDeviceToHost));
+    HIPCHK(hipGraphAddMemcpyNode1D(&cw, e->rg_graph, &cp, 1, e->h_wave_busy.p, e->wave_busy.p,
+                                   (size_t)e->wave_slots * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipGraphInstantiate(&e->rg_exec, e->rg_graph, nullptr, nullptr, 0));
     e->rg_key = key;
   } else {
@@ -628,6 +641,8 @@ int dfs_round(dsm_engine *e, const RoundPart *parts, int nparts, hipEvent_t ev0,
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
     HIPCHK(launch_dfs(dfs_args(e), dfs_grid(e, nrec), e->dfs_wpb, e->max_qn, e->stream));
     HIPCHK(hipMemcpyAsync(e->h_ctl.p, e->ctl.p, sizeof(DfsCtl), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(e->h_wave_busy.p, e->wave_busy.p, (size_t)e->wave_slots * 8, hipMemcpyDeviceToHost,
+                          e->stream));
   }
   if (ev2) HIPCHK(hipEventRecord(ev2, e->stream));
   return DSM_OK;
@@ -651,6 +666,23 @@ int check_round(dsm_engine *e, uint64_t *seeds) {
   e->stats.dfs_entries += c.scanned.v;
   e->stats.dfs_search_loads += c.loads.v;
   e->stats.dfs_merge_bytes += c.mbytes.v;
+  // balance: per-block task time of the launch
+  {
+    const uint32_t nb = e->last_blocks, wpb = e->last_wpb;
+    if (nb && (uint64_t)nb * wpb <= e->wave_slots) {
+      double tot = 0, mx = 0;
+      for (uint32_t b = 0; b < nb; b++) {
+        double s = 0;
+        for (uint32_t w = 0; w < wpb; w++) s += (double)e->h_wave_busy.p[(size_t)b * wpb + w];
+        tot += s;
+        mx = std::max(mx, s);
+      }
+      const double to_ms = 1.0 / (double)e->wall_khz;
+      e->stats.dfs_busy_ms += tot * to_ms;
+      e->stats.dfs_waves += (uint64_t)nb * wpb;
+      if (tot > 0) e->stats.dfs_block_imbalance += mx / (tot / nb);
+    }
+  }
   e->stats.dfs_launches += 1;
   return DSM_OK;
 }
@@ -909,8 +941,19 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   int per_cu = dfs_max_blocks_per_cu(wpb, e->max_qn);
   if (e->opt.blocks_per_cu && (int)e->opt.blocks_per_cu < per_cu) per_cu = (int)e->opt.blocks_per_cu;
   e->dfs_blocks = (uint32_t)(per_cu * e->num_cu);
+  // room for the largest grid any launch shape can have (the autotuner may
+  // pick another shape later)
+  {
+    uint32_t most = 0;
+    for (uint32_t w : {1u, 2u, 4u, 8u}) most = std::max<uint32_t>(most, (uint32_t)dfs_max_blocks_per_cu(w, e->max_qn) * w);
+    e->wave_slots = most * (uint32_t)e->num_cu;
+    HIPCHK(e->wave_busy.alloc(e->wave_slots));
+    HIPCHK(e->h_wave_busy.ensure(e->wave_slots));
+  }
   e->donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
   e->lane_work = e->opt.lane_work ? e->opt.lane_work : 16;
+  e->donate_every = e->opt.donate_every ? e->opt.donate_every : 32;
+  e->donate_min_n2 = e->opt.donate_min_n2 ? e->opt.donate_min_n2 : 64;
   e->leaf_sets = e->opt.leaf_sets != 0;
   e->graph_rounds = e->opt.graph_rounds != 0;
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));

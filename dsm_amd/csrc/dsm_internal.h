@@ -142,6 +142,8 @@ struct DfsArgs {
   uint32_t epoch;
   uint32_t donate_min;
   uint32_t lane_work;            // longest leaf list one lane scans alone
+  uint32_t donate_every;         // chunks between checks for waiting waves
+  uint32_t donate_min_n2;        // smallest range split off one level above the leaf
   uint32_t leaf_sets;            // fused leaf level on/off
   uint32_t nq;                   // queries (count layout)
   const uint32_t *seed_tag;      // per seed: count slot << 1 | sign
@@ -151,6 +153,8 @@ struct DfsArgs {
   uint32_t *tuples;              // [tuple_cap][kMaxQ+2] or null
   unsigned long long tuple_cap;
   unsigned long long *tuple_count;  // tuples found this batch
+  unsigned long long *wave_busy;    // per wave of the grid: wall-clock ticks on tasks (or null)
+  uint32_t wave_slots;              // entries of wave_busy
 };
 
 struct SeedArgs {

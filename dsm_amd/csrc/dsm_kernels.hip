@@ -22,7 +22,6 @@ namespace {
 
 constexpr int kApplyWaves = 4;
 constexpr int kLight = 64;          // dirty vertices up to this degree: one lane each
-constexpr uint32_t kDonateEvery = 32;  // chunks between checks for waiting waves
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
@@ -793,12 +792,14 @@ __device__ __forceinline__ bool lds_contains(const uint32_t *a, uint32_t n, uint
   return a[lo] == x;
 }
 
-// Tests source entries [p, min(p + 64, e)) of `src`, one per lane: the
-// candidate bit of signature `sig`, injectivity against the positions in
-// `inj`, then membership in the list of every position in `oth`. The source
-// chunk is sorted, so the entries still alive span [wmin, wmax]; the wave
-// finds that window of the other list cooperatively (bounds2) and, when it
-// holds at most kWin entries, stages it in LDS for the lanes' searches.
+// Tests source entries [p, min(p + 64, e)) of `src`, one per lane:
+// injectivity against the positions in `inj`, membership in the list of every
+// position in `oth`, then the candidate bit of signature `sig` for the
+// survivors only (the bitmap words are the scattered reads; the membership
+// windows are contiguous). The source chunk is sorted, so the entries still
+// alive span [wmin, wmax]; the wave finds that window of the other list
+// cooperatively (bounds2) and, when it holds at most kWin entries, stages it
+// in LDS for the lanes' searches.
 __device__ __forceinline__ bool test_entries(const WaveMem &M, const DfsArgs &a, const uint32_t *src, uint32_t oth,
                                              uint32_t inj, uint32_t sig, uint32_t p, uint32_t e, uint32_t &w,
                                              uint32_t &loads) {
@@ -1043,7 +1044,7 @@ __device__ void try_donate(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l
     const uint32_t unf = e > p ? e - p : 0;
     if (k == l) {
       // entries one level above the leaf carry a leaf count each
-      const uint32_t dmin = (k + 2 == n && a.donate_min > 64) ? 64u : a.donate_min;
+      const uint32_t dmin = k + 2 == n ? a.donate_min_n2 : a.donate_min;
       if (unf >= dmin) {
         lv = k;
         mid = p + unf / 2;
@@ -1144,6 +1145,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
   uint32_t ticket = 0xFFFFFFFFu;
   uint32_t naps = 0;            // polls of the current ticket (backoff)
   uint64_t idle_since = 0;      // wall clock when the wave ran out of work (watchdog)
+  uint64_t busy = 0;            // wall-clock ticks spent on tasks
   for (;;) {
     // lane 0 claims work; every decision is then broadcast so that all
     // control flow below stays wave-uniform
@@ -1201,6 +1203,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
       continue;
     }
     // ---- load the task
+    const uint64_t t_task = wall_clock64();
     uint32_t plan, L0, lo, hi, tag, fw = 0;
     if (got == 1) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1261,7 +1264,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
         if (l == n - 1) {
           // leaf: count every valid image in the remaining source range
           for (;;) {
-            if (++tick >= kDonateEvery) {
+            if (++tick >= a.donate_every) {
               tick = 0;
               try_donate(S, a, L0, l, n);
             }
@@ -1307,7 +1310,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
           // per candidate
           bool large = false;
           for (;;) {
-            if (++tick >= kDonateEvery) {
+            if (++tick >= a.donate_every) {
               tick = 0;
               try_donate(S, a, L0, l, n);
             }
@@ -1349,7 +1352,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
             continue;
           }
         }
-        if (++tick >= kDonateEvery) {
+        if (++tick >= a.donate_every) {
           tick = 0;
           try_donate(S, a, L0, l, n);
         }
@@ -1404,7 +1407,11 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
       atomicAdd(&a.ctl->mbytes.v, (unsigned long long)mb);
       atomicSub(&a.ctl->work_left.v, 1);
     }
+    busy += wall_clock64() - t_task;
   }
+  // time this wave spent on tasks (not waiting), for the balance statistics
+  const uint32_t slot_w = blockIdx.x * WPB + (threadIdx.x >> 6);
+  if (a.wave_busy && lane == 0 && slot_w < a.wave_slots) a.wave_busy[slot_w] = busy;
 }
 
 // Copies every vertex's live list to its new slot (pool growth/compaction).

@@ -66,7 +66,8 @@ class Options(C.Structure):
                 ("tuple_capacity", C.c_uint64), ("tuples_in_host_memory", C.c_int),
                 ("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("donate_min", C.c_uint32), ("queue_capacity", C.c_uint32), ("watchdog_ms", C.c_uint32),
-                ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("graph_rounds", C.c_int)]
+                ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("donate_every", C.c_uint32),
+                ("donate_min_n2", C.c_uint32), ("graph_rounds", C.c_int)]
 
 
 class TuneReport(C.Structure):
@@ -86,7 +87,8 @@ class Stats(C.Structure):
                 ("ms_seed", C.c_float), ("ms_dfs_pos", C.c_float), ("ms_dfs_neg", C.c_float),
                 ("pool_entries_used", C.c_uint64), ("pool_entries_capacity", C.c_uint64),
                 ("dfs_entries", C.c_uint64), ("dfs_search_loads", C.c_uint64), ("dfs_launches", C.c_uint32),
-                ("dfs_merge_bytes", C.c_uint64)]
+                ("dfs_merge_bytes", C.c_uint64), ("dfs_busy_ms", C.c_double), ("dfs_waves", C.c_uint64),
+                ("dfs_block_imbalance", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -243,7 +245,8 @@ class Engine:
 
     def __init__(self, graph, queries, device=0, relabel=True, count_negatives=True, tuple_capacity=0,
                  tuples_in_host_memory=False, waves_per_block=4, blocks_per_cu=0, donate_min=0,
-                 queue_capacity=0, watchdog_ms=0, lane_work=0, leaf_sets=True, graph_rounds=True):
+                 queue_capacity=0, watchdog_ms=0, lane_work=0, leaf_sets=True, graph_rounds=True,
+                 donate_every=0, donate_min_n2=0):
         L = lib()
         n, lab, eu, ev = graph
         self._keep = []
@@ -270,6 +273,8 @@ class Engine:
         opt.lane_work = lane_work
         opt.leaf_sets = int(bool(leaf_sets))
         opt.graph_rounds = int(bool(graph_rounds))
+        opt.donate_every = donate_every
+        opt.donate_min_n2 = donate_min_n2
         h = C.c_void_p()
         rc = L.dsm_create(C.byref(gd), qds, len(queries), C.byref(opt), C.byref(h))
         self._keep = None

@@ -84,6 +84,9 @@ typedef struct {
   int leaf_sets;             /* 1: one level above the leaf, candidates count their
                                 leaf images against a leaf set shared by the parent;
                                 0: one descent per candidate */
+  uint32_t donate_every;     /* scan steps between a busy wave's checks for idle waves (0 = 32) */
+  uint32_t donate_min_n2;    /* smallest range one level above the leaf that is split
+                                off to an idle wave (0 = 64) */
   int graph_rounds;          /* 1: each DFS round (control reset, seeding, DFS, status
                                 copy) is one HIP graph launch, built once and updated
                                 per round; 0: separate launches */
@@ -107,6 +110,12 @@ typedef struct {
   /* 4 B x degree of every backward list of every partial match the DFS
    * expanded: the bytes merge intersections over whole lists would read */
   uint64_t dfs_merge_bytes;
+  /* balance of the persistent DFS grid, summed over the launches: time the
+   * waves spent on tasks (ms, all waves), waves launched, and per launch the
+   * busiest block's task time over the mean block's */
+  double dfs_busy_ms;
+  uint64_t dfs_waves;
+  double dfs_block_imbalance;
 } dsm_stats;
 
 /* Outcome of dsm_autotune: the launch parameters kept and the DFS time of the
