@@ -68,7 +68,11 @@ inline uint64_t ekey(uint32_t a, uint32_t b) {
 
 }  // namespace
 
+static int g_progress = 0;  // > 0: long generators report progress on stderr
+
 extern "C" {
+
+void gen_set_progress(int on) { g_progress = on; }
 
 // G(n,m): m distinct undirected edges, no self loops, uniform over pairs.
 int64_t gen_edges_uniform(uint32_t n, uint64_t m, uint64_t seed, uint32_t *eu, uint32_t *ev) {
@@ -116,6 +120,10 @@ int64_t gen_edges_rmat(uint32_t n, uint64_t m, double a, double b, double c, uin
     }
     if (u >= n || v >= n || u == v || !set.insert(ekey(u, v))) continue;
     eu[k] = u; ev[k] = v; k++;
+    if (g_progress && (k & ((1ull << 25) - 1)) == 0) {
+      fprintf(stderr, "R-MAT: %llu of %llu edges\n", (unsigned long long)k, (unsigned long long)m);
+      fflush(stderr);
+    }
   }
   std::vector<uint32_t> perm(n);
   for (uint32_t i = 0; i < n; i++) perm[i] = i;

@@ -52,6 +52,9 @@ def _lib():
     L.gen_edges_rmat.argtypes = [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,
                                  C.c_uint64, u32p, u32p]
     L.gen_labels.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_uint64, u32p]
+    L.gen_set_progress.argtypes = [C.c_int]
+    # graphs of more than 2^25 edges report generation progress on stderr
+    L.gen_set_progress(1)
     L.gen_permutation.argtypes = [C.c_uint64, C.c_uint64, u64p]
     L.gen_csr.argtypes = [C.c_uint32, C.c_uint64, u32p, u32p, u64p, u32p]
     L.gen_find_pattern.restype = C.c_int
