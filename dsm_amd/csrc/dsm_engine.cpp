@@ -137,6 +137,7 @@ struct dsm_engine {
   uint32_t donate_min = 256, lane_work = 16, donate_every = 32, donate_min_n2 = 64;
   bool leaf_sets = true;
   bool graph_rounds = true;
+  bool leaf_segments = true;
   // per-wave task time of the last DFS launch (balance statistics)
   DevBuf<unsigned long long> wave_busy;
   PinnedBuf<unsigned long long> h_wave_busy;
@@ -554,6 +555,7 @@ DfsArgs dfs_args(dsm_engine *e) {
   da.lane_work = e->lane_work;
   da.donate_every = e->donate_every;
   da.donate_min_n2 = e->donate_min_n2;
+  da.leaf_segments = e->leaf_segments ? 1u : 0u;
   da.leaf_sets = e->leaf_sets ? 1u : 0u;
   da.nq = (uint32_t)e->queries.size();
   da.watchdog = (unsigned long long)e->wall_khz * e->watchdog_ms;
@@ -956,6 +958,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   e->donate_min_n2 = e->opt.donate_min_n2 ? e->opt.donate_min_n2 : 64;
   e->leaf_sets = e->opt.leaf_sets != 0;
   e->graph_rounds = e->opt.graph_rounds != 0;
+  e->leaf_segments = e->opt.leaf_segments != 0;
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
   HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -976,6 +979,7 @@ void dsm_options_default(dsm_options *opt) {
   opt->waves_per_block = 4;
   opt->leaf_sets = 1;
   opt->graph_rounds = 1;
+  opt->leaf_segments = 1;
 }
 
 const char *dsm_last_error(const dsm_engine *e) {
@@ -1193,7 +1197,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
   if ((rc = upload_records(e, s, d, rb))) return rc;
   struct Cfg {
     uint32_t wpb, bpc, dmin, lw;
-    bool ls;
+    bool ls, seg;
   };
   auto max_bpc = [&](uint32_t wpb) { return (uint32_t)std::max(1, dfs_max_blocks_per_cu(wpb, e->max_qn)); };
   // times one sample round under c and checks its counts against the first
@@ -1205,6 +1209,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     e->donate_min = c.dmin;
     e->lane_work = c.lw;
     e->leaf_sets = c.ls;
+    e->leaf_segments = c.seg;
     HIPCHK(launch_ctl_reset(nullptr, nullptr, e->tuple_count.p, e->counts.p, nq * 2, e->stream));
     int rc2 = run_round(e, rb.p, rb.p + k, k, 0, e->ev[9], e->ev[10], e->ev[11]);
     if (rc2) return rc2;
@@ -1219,7 +1224,8 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
       return e->fail_msg(DSM_EDEVICE, "autotune: launch configurations disagree on the counts");
     return DSM_OK;
   };
-  Cfg best{e->dfs_wpb, e->dfs_blocks / (uint32_t)e->num_cu, e->donate_min, e->lane_work, e->leaf_sets};
+  Cfg best{e->dfs_wpb, e->dfs_blocks / (uint32_t)e->num_cu, e->donate_min, e->lane_work, e->leaf_sets,
+           e->leaf_segments};
   // warm-up: clocks and caches settle before anything is compared
   float best_ms = 1e30f;
   for (int r = 0; r < 4; r++) {
@@ -1288,6 +1294,9 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     c.ls = !best.ls;
     if ((rc = consider(c))) return rc;
     if (best.ls) {
+      Cfg c3 = best;
+      c3.seg = !best.seg;
+      if ((rc = consider(c3))) return rc;
       const Cfg base = best;
       for (uint32_t v : {8u, 16u, 24u, 48u, 64u}) {
         if (v == base.lw) continue;
@@ -1310,6 +1319,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     rep->donate_min = best.dmin;
     rep->lane_work = best.lw;
     rep->leaf_sets = best.ls ? 1 : 0;
+    rep->leaf_segments = best.seg ? 1 : 0;
     rep->trials = trials;
     rep->ms_initial = ms0;
     rep->ms_best = best_ms;

@@ -921,7 +921,7 @@ __device__ __forceinline__ bool leaf_accept(const WaveHdr &S, const DfsArgs &a, 
 }
 
 __device__ __forceinline__ uint32_t leaf_count_lane(const WaveMem &M, const DfsArgs &a, uint32_t leaf, uint32_t w,
-                                                    bool &defer, LeafList &out, uint32_t &scanned,
+                                                    bool &defer, bool &scan, LeafList &out, uint32_t &scanned,
                                                     uint32_t &loads, uint32_t &wdeg) {
   const WaveHdr &S = *M.h;
   const uint32_t pv = leaf - 1;
@@ -948,6 +948,14 @@ __device__ __forceinline__ uint32_t leaf_count_lane(const WaveMem &M, const DfsA
   }
   const uint32_t lg = 33u - (uint32_t)__builtin_clz(len | 1u);
   const bool by_f = st == kFReady && (uint64_t)fn * lg < len;
+  if (!by_f && a.leaf_segments) {
+    // scanned with every other lane's list as one segmented array (leaf_scan_segments)
+    scan = true;
+    out.L = L;
+    out.len = len;
+    out.by_f = false;
+    return 0;
+  }
   const uint32_t work = by_f ? fn * lg : len;
   if (work > a.lane_work) {
     defer = true;
@@ -966,6 +974,47 @@ __device__ __forceinline__ uint32_t leaf_count_lane(const WaveMem &M, const DfsA
   } else {
     for (uint32_t k = 0; k < len; k++) c += leaf_accept(S, a, M, leaf, st, fn, injw, w, L[k]) ? 1u : 0u;
     scanned += len;
+  }
+  return c;
+}
+
+// The lists of the lanes with `has`, scanned as one segmented array: entry k
+// of the concatenation is read by lane k mod 64, which finds the list it falls
+// in by a binary search over the lanes' exclusive prefix sums (shuffles). Short
+// lists thus share each 64-entry step instead of keeping their lanes in a
+// serial loop, and consecutive entries of a list are read by consecutive
+// lanes. Returns this lane's share of the count (the task sums the wave).
+__device__ __forceinline__ uint32_t leaf_scan_segments(const WaveMem &M, const DfsArgs &a, uint32_t leaf, uint32_t w,
+                                                       bool has, const LeafList &ll, uint32_t &scanned) {
+  const WaveHdr &S = *M.h;
+  const uint32_t lane = lane_id();
+  const bool injw = (S.inj[leaf] >> (leaf - 1)) & 1;
+  const uint32_t st = S.fstate, fn = S.fn;
+  const uint32_t n = has ? ll.len : 0u;
+  uint32_t incl = n;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o);
+    if (lane >= (uint32_t)o) incl += y;
+  }
+  const uint32_t total = __shfl(incl, 63);
+  const uint32_t excl = incl - n;
+  const unsigned long long lp = has ? (unsigned long long)ll.L : 0ull;
+  uint32_t c = 0;
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t k = base + lane;
+    // the owner is the last lane whose list starts at or before k
+    uint32_t o = 0;
+    for (uint32_t step = 32; step; step >>= 1) {
+      const uint32_t cand = o + step;
+      if (__shfl(excl, (int)cand) <= k) o = cand;
+    }
+    const uint32_t start = __shfl(excl, (int)o);
+    const uint32_t *L = (const uint32_t *)__shfl(lp, (int)o);
+    const uint32_t wo = __shfl(w, (int)o);
+    if (k < total) {
+      c += leaf_accept(S, a, M, leaf, st, fn, injw, wo, L[k - start]) ? 1u : 0u;
+      scanned++;
+    }
   }
   return c;
 }
@@ -1334,13 +1383,14 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
               large = true;
               break;
             }
-            bool defer = false;
+            bool defer = false, scan = false;
             LeafList ll{nullptr, 0, false};
             uint32_t wdeg = 0;
             if (ok) {
-              lcount += leaf_count_lane(M, a, l + 1, w, defer, ll, scanned, loads, wdeg);
+              lcount += leaf_count_lane(M, a, l + 1, w, defer, scan, ll, scanned, loads, wdeg);
               mbytes += fixed_leaf_bytes + 4ull * wdeg;
             }
+            if (a.leaf_segments) lcount += leaf_scan_segments(M, a, l + 1, w, scan, ll, scanned);
             lcount += leaf_count_wave(M, a, l + 1, w, defer, ll, scanned, loads);
             wave_sync();
             if (lane == 0) S.pos[l] = p + 64;

@@ -87,6 +87,8 @@ typedef struct {
   uint32_t donate_every;     /* scan steps between a busy wave's checks for idle waves (0 = 32) */
   uint32_t donate_min_n2;    /* smallest range one level above the leaf that is split
                                 off to an idle wave (0 = 64) */
+  int leaf_segments;         /* 1: with leaf sets, the lanes' leaf lists are scanned as
+                                one segmented array; 0: each lane scans its own */
   int graph_rounds;          /* 1: each DFS round (control reset, seeding, DFS, status
                                 copy) is one HIP graph launch, built once and updated
                                 per round; 0: separate launches */
@@ -122,7 +124,7 @@ typedef struct {
  * sample round before and after. */
 typedef struct {
   uint32_t waves_per_block, blocks_per_cu, donate_min, lane_work;
-  int leaf_sets;
+  int leaf_sets, leaf_segments;
   uint32_t trials;        /* configurations timed */
   float ms_initial, ms_best;
 } dsm_tune_report;
