@@ -754,10 +754,14 @@ int apply_round(dsm_engine *e, const HostDelta &d, const std::vector<uint32_t> &
   // layout: tv | ioff | doff | ins | del
   const size_t o_tv = 0, o_io = o_tv + T, o_do = o_io + T + 1, o_in = o_do + T + 1, o_de = o_in + ins.size();
   const size_t total = o_de + del.size();
+  // device-only space after the uploaded part: prefix counts and big-list
+  // indices of the hub lists that the whole grid copies
+  const size_t o_gi = total, o_gd = o_gi + ins.size() + T, o_bm = o_gd + del.size() + T, o_bc = o_bm + T;
+  const size_t dtotal = o_bc + T;
   // queued work must be done with the staging and device buffers before they are reused
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(e->hbuf.ensure(total));
-  HIPCHK(e->bbuf.ensure(total));
+  HIPCHK(e->bbuf.ensure(dtotal));
   uint32_t *h = e->hbuf.p;
   std::copy(tv.begin(), tv.end(), h + o_tv);
   std::copy(ioff.begin(), ioff.end(), h + o_io);
@@ -778,6 +782,10 @@ int apply_round(dsm_engine *e, const HostDelta &d, const std::vector<uint32_t> &
   b.ndeg = e->ndeg.p;
   b.soff = e->soff.p;
   b.fail = e->fail.p;
+  b.gicum = e->bbuf.p + o_gi;
+  b.gdcum = e->bbuf.p + o_gd;
+  b.big_merge = e->bbuf.p + o_bm;
+  b.big_commit = e->bbuf.p + o_bc;
   // scratch must hold every touched list once plus the inserts
   HIPCHK(launch_scratch_reset(e->actl.p, e->stream));
   HIPCHK(launch_apply_merge(e->dev_graph(), b, e->scratch_cap, e->scratch.p, e->actl.p, e->stream));
@@ -869,13 +877,14 @@ int do_apply_delta(dsm_engine *e, const HostDelta &d) {
   if (R) {
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(e->hbuf.ensure(2 * (size_t)R));
-    HIPCHK(e->bbuf.ensure(2 * (size_t)R));
+    HIPCHK(e->bbuf.ensure(3 * (size_t)R));
     std::copy(d.rl_v.begin(), d.rl_v.end(), e->hbuf.p);
     std::copy(d.rl_l.begin(), d.rl_l.end(), e->hbuf.p + R);
     HIPCHK(hipMemcpyAsync(e->bbuf.p, e->hbuf.p, 2 * (size_t)R * 4, hipMemcpyHostToDevice, e->stream));
     b.nrelabel = R;
     b.rl_v = e->bbuf.p;
     b.rl_l = e->bbuf.p + R;
+    b.rl_big = e->bbuf.p + 2 * (size_t)R;
     HIPCHK(launch_relabel(e->dev_graph(), b, e->tables(), e->dirty(), e->stream));
   }
   return DSM_OK;

@@ -84,7 +84,10 @@ struct alignas(64) ApplyCtl {
   uint32_t ndirty;                  // vertices on the dirty list
   uint32_t nheavy;                  // dirty vertices re-filtered by a whole wave
   uint32_t relabeled;
-  uint32_t pad[7];
+  uint32_t nrelabel_big;            // relabelled vertices whose neighbours the whole grid marks
+  uint32_t nmerge_big;              // touched vertices whose merge the whole grid copies
+  uint32_t ncommit_big;             // touched vertices whose commit the whole grid copies
+  uint32_t pad[4];
 };
 
 struct DevGraph {
@@ -124,9 +127,17 @@ struct BatchDev {
   uint32_t *ndeg;           // [ntouched] new degree (merge -> commit)
   unsigned long long *soff; // [ntouched] scratch offset
   uint32_t *fail;           // [ntouched] commit found no room
+  // lists longer than kApplyWave: their merge and commit copies are split over
+  // the whole grid. icum/dcum of touched vertex t live at ioff[t] + t and
+  // doff[t] + t; big_merge / big_commit list the vertices (indices t).
+  uint32_t *gicum;          // [ins + ntouched]
+  uint32_t *gdcum;          // [del + ntouched]
+  uint32_t *big_merge;      // [ntouched]
+  uint32_t *big_commit;     // [ntouched]
   uint32_t nrelabel;
   const uint32_t *rl_v;
   const uint32_t *rl_l;     // device label (already clamped)
+  uint32_t *rl_big;         // [nrelabel] relabelled vertices of degree > kRelabelWave
 };
 
 struct DfsArgs {

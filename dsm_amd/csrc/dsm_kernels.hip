@@ -21,6 +21,7 @@ namespace dsm {
 namespace {
 
 constexpr int kApplyWaves = 4;
+constexpr uint32_t kApplyWave = 8192;  // longer lists are copied by the whole grid
 constexpr int kLight = 64;          // dirty vertices up to this degree: one lane each
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -144,6 +145,11 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
     }
     wave_sync();
     const uint32_t newdeg = na - nhit + nnew;
+    if (na > kApplyWave) {
+      // the prefix counts go to global memory; k_apply_merge_big copies
+      for (uint32_t j = lane; j <= ni; j += 64) b.gicum[i0 + t + j] = icum[j];
+      for (uint32_t j = lane; j <= nd; j += 64) b.gdcum[d0 + t + j] = dcum[j];
+    }
     unsigned long long so = 0;
     if (lane == 0) so = atomicAdd(&actl->scratch_used, (unsigned long long)(na + nnew));
     so = bcast64(so);
@@ -156,6 +162,15 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
       continue;
     }
     uint32_t *out = scratch + so;
+    if (na > kApplyWave) {
+      if (lane == 0) {
+        b.ndeg[t] = newdeg;
+        b.soff[t] = so;
+        b.big_merge[atomicAdd(&actl->nmerge_big, 1u)] = t;
+      }
+      wave_sync();
+      continue;
+    }
     for (uint32_t j = lane; j < na; j += 64) {
       const uint32_t a = A[j];
       const uint32_t p = lower_bound(D, nd, a);
@@ -175,6 +190,41 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
       b.soff[t] = so;
     }
     wave_sync();
+  }
+}
+
+// The merge copies of the lists longer than kApplyWave, each split evenly over
+// every thread of the grid, with the prefix counts k_apply_merge stored.
+__global__ __launch_bounds__(256) void k_apply_merge_big(DevGraph g, BatchDev b, uint32_t *scratch,
+                                                         const ApplyCtl *actl) {
+  const uint32_t nbig = actl->nmerge_big;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+  for (uint32_t r = 0; r < nbig; r++) {
+    const uint32_t t = b.big_merge[r];
+    const uint32_t x = b.tv[t];
+    const uint32_t *A = g.pool + g.start[x];
+    const uint32_t na = g.deg[x];
+    const uint32_t i0 = b.ioff[t], ni = b.ioff[t + 1] - i0;
+    const uint32_t d0 = b.doff[t], nd = b.doff[t + 1] - d0;
+    const uint32_t *I = b.ins + i0;
+    const uint32_t *D = b.del + d0;
+    const uint32_t *icum = b.gicum + i0 + t;
+    const uint32_t *dcum = b.gdcum + d0 + t;
+    uint32_t *out = scratch + b.soff[t];
+    for (uint32_t j = tid; j < na; j += nth) {
+      const uint32_t a = A[j];
+      const uint32_t p = lower_bound(D, nd, a);
+      if (p < nd && D[p] == a) continue;
+      const uint32_t q = lower_bound(I, ni, a);
+      out[j - dcum[p] + icum[q]] = a;
+    }
+    for (uint32_t j = tid; j < ni; j += nth) {
+      if (icum[j + 1] == icum[j]) continue;
+      const uint32_t i = I[j];
+      const uint32_t rr = lower_bound(A, na, i);
+      const uint32_t p = lower_bound(D, nd, i);
+      out[rr - dcum[p] + icum[j]] = i;
+    }
   }
 }
 
@@ -214,7 +264,12 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_commit(DevGraph g, B
         atomicAdd(&actl->relocated, 1u);
       }
     }
-    for (uint32_t j = lane; j < nd; j += 64) g.pool[dst + j] = scratch[so + j];
+    if (nd > kApplyWave) {
+      // k_apply_commit_big copies; the new degree is only read by later kernels
+      if (lane == 0) b.big_commit[atomicAdd(&actl->ncommit_big, 1u)] = t;
+    } else {
+      for (uint32_t j = lane; j < nd; j += 64) g.pool[dst + j] = scratch[so + j];
+    }
     if (lane == 0) {
       g.deg[x] = nd;
       b.fail[t] = 0;
@@ -223,9 +278,33 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_commit(DevGraph g, B
   }
 }
 
+// The commit copies of the lists longer than kApplyWave, split over the grid.
+__global__ __launch_bounds__(256) void k_apply_commit_big(DevGraph g, BatchDev b, const uint32_t *scratch,
+                                                          const ApplyCtl *actl) {
+  const uint32_t nbig = actl->ncommit_big;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+  for (uint32_t r = 0; r < nbig; r++) {
+    const uint32_t t = b.big_commit[r];
+    const uint32_t x = b.tv[t];
+    const uint32_t dst = g.start[x];
+    const uint32_t nd = b.ndeg[t];
+    const unsigned long long so = b.soff[t];
+    for (uint32_t j = tid; j < nd; j += nth) g.pool[dst + j] = scratch[so + j];
+  }
+}
+
+__global__ void k_big_reset(ApplyCtl *actl, int merge, int commit) {
+  if (threadIdx.x == 0) {
+    if (merge) actl->nmerge_big = 0;
+    if (commit) actl->ncommit_big = 0;
+  }
+}
+
 // One wave per relabel entry: set the label; when it changed, clear the
 // vertex's bits in every bitmap of its old label and mark it and all its
 // (post-update) neighbours dirty, since their neighbour-label counts changed.
+constexpr uint32_t kRelabelWave = 2048;  // longer lists are marked by the whole grid (k_relabel_big)
+
 __global__ __launch_bounds__(256) void k_relabel(DevGraph g, BatchDev b, FilterTables ft, DirtyArgs d) {
   const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
   for (uint32_t r = blockIdx.x * 4 + wv; r < b.nrelabel; r += gridDim.x * 4) {
@@ -245,7 +324,24 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, BatchDev b, FilterT
     }
     const uint32_t *A = g.pool + g.start[x];
     const uint32_t na = g.deg[x];
+    if (na > kRelabelWave) {
+      if (lane == 0) b.rl_big[atomicAdd(&d.actl->nrelabel_big, 1u)] = x;
+      continue;
+    }
     for (uint32_t j = lane; j < na; j += 64) mark_dirty(d, A[j]);
+  }
+}
+
+// The neighbours of every relabelled vertex of degree > kRelabelWave, marked
+// dirty by the whole grid: each list is split evenly over all threads.
+__global__ __launch_bounds__(256) void k_relabel_big(DevGraph g, BatchDev b, DirtyArgs d) {
+  const uint32_t nbig = d.actl->nrelabel_big;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+  for (uint32_t r = 0; r < nbig; r++) {
+    const uint32_t x = b.rl_big[r];
+    const uint32_t *A = g.pool + g.start[x];
+    const uint32_t na = g.deg[x];
+    for (uint32_t j = tid; j < na; j += nth) mark_dirty(d, A[j]);
   }
 }
 
@@ -625,6 +721,7 @@ __global__ void k_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tup
       actl->ndirty = 0;
       actl->nheavy = 0;
       actl->relabeled = 0;
+      actl->nrelabel_big = 0;
     }
   }
 }
@@ -1491,8 +1588,12 @@ hipError_t launch_apply_merge(const DevGraph &g, const BatchDev &b, unsigned lon
                               uint32_t *scratch, ApplyCtl *actl, hipStream_t s) {
   if (b.ntouched == 0) return hipSuccess;
   const uint32_t blocks = (b.ntouched + kApplyWaves - 1) / kApplyWaves;
+  hipLaunchKernelGGL(k_big_reset, dim3(1), dim3(64), 0, s, actl, 1, 0);
   hipLaunchKernelGGL(k_apply_merge, dim3(blocks < 4096 ? blocks : 4096), dim3(kApplyWaves * 64), 0, s, g, b,
                      scratch_cap, scratch, actl);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_apply_merge_big, dim3(1024), dim3(256), 0, s, g, b, scratch, actl);
   return hipGetLastError();
 }
 
@@ -1501,8 +1602,12 @@ hipError_t launch_apply_commit(const DevGraph &g, const BatchDev &b, const uint3
                                hipStream_t s, int only_failed) {
   if (b.ntouched == 0) return hipSuccess;
   const uint32_t blocks = (b.ntouched + kApplyWaves - 1) / kApplyWaves;
+  hipLaunchKernelGGL(k_big_reset, dim3(1), dim3(64), 0, s, actl, 0, 1);
   hipLaunchKernelGGL(k_apply_commit, dim3(blocks < 4096 ? blocks : 4096), dim3(kApplyWaves * 64), 0, s, g,
                      b, scratch, pool_cap, actl, d, only_failed);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_apply_commit_big, dim3(1024), dim3(256), 0, s, g, b, scratch, actl);
   return hipGetLastError();
 }
 
@@ -1523,6 +1628,9 @@ hipError_t launch_relabel(const DevGraph &g, const BatchDev &b, const FilterTabl
   if (b.nrelabel == 0) return hipSuccess;
   const uint32_t blocks = (b.nrelabel + 3) / 4;
   hipLaunchKernelGGL(k_relabel, dim3(blocks < 4096 ? blocks : 4096), dim3(256), 0, s, g, b, ft, d);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_relabel_big, dim3(1024), dim3(256), 0, s, g, b, d);
   return hipGetLastError();
 }
 
