@@ -92,6 +92,9 @@ __device__ __forceinline__ bool ge_bytes(uint64_t x, uint64_t y) {
 
 __device__ __forceinline__ void mark_dirty(const DirtyArgs &d, uint32_t x) {
   const uint32_t bit = 1u << (x & 31);
+  // a set bit stays set for the batch: read first, so that marking an already
+  // dirty vertex (the common case around hubs) costs a load, not an atomic
+  if (ld_relaxed(&d.bits[x >> 5]) & bit) return;
   const uint32_t old = atomicOr(&d.bits[x >> 5], bit);
   if (!(old & bit)) {
     const uint32_t p = atomicAdd(&d.actl->ndirty, 1u);
@@ -198,6 +201,8 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
 __global__ __launch_bounds__(256) void k_apply_merge_big(DevGraph g, BatchDev b, uint32_t *scratch,
                                                          const ApplyCtl *actl) {
   const uint32_t nbig = actl->nmerge_big;
+  const uint32_t lane = lane_id();
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
   for (uint32_t r = 0; r < nbig; r++) {
     const uint32_t t = b.big_merge[r];
@@ -211,12 +216,23 @@ __global__ __launch_bounds__(256) void k_apply_merge_big(DevGraph g, BatchDev b,
     const uint32_t *icum = b.gicum + i0 + t;
     const uint32_t *dcum = b.gdcum + d0 + t;
     uint32_t *out = scratch + b.soff[t];
-    for (uint32_t j = tid; j < na; j += nth) {
-      const uint32_t a = A[j];
-      const uint32_t p = lower_bound(D, nd, a);
-      if (p < nd && D[p] == a) continue;
-      const uint32_t q = lower_bound(I, ni, a);
-      out[j - dcum[p] + icum[q]] = a;
+    // waves take 64-entry chunks of A in turn; the deletes and inserts that
+    // fall inside a chunk's key range are found once per chunk, so each entry
+    // searches only that (usually empty) range
+    for (uint32_t c0 = gw * 64; c0 < na; c0 += nw * 64) {
+      const uint32_t c1 = c0 + 64 < na ? c0 + 64 : na;
+      const uint32_t lo_key = A[c0], hi_key = A[c1 - 1];
+      const uint32_t p0 = lower_bound(D, nd, lo_key), p1 = lower_bound(D, nd, hi_key + 1);
+      const uint32_t q0 = lower_bound(I, ni, lo_key), q1 = lower_bound(I, ni, hi_key + 1);
+      const uint32_t j = c0 + lane;
+      if (j < c1) {
+        const uint32_t a = A[j];
+        const uint32_t p = p0 + lower_bound(D + p0, p1 - p0, a);
+        if (!(p < nd && D[p] == a)) {
+          const uint32_t q = q0 + lower_bound(I + q0, q1 - q0, a);
+          out[j - dcum[p] + icum[q]] = a;
+        }
+      }
     }
     for (uint32_t j = tid; j < ni; j += nth) {
       if (icum[j + 1] == icum[j]) continue;
