@@ -295,7 +295,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--served-rates", default="1e4,3e4,5e4,7e4,1e5,3e5,1e6,1e7")
+    ap.add_argument("--served-rates", default="1e4,3e4,5e4,7e4,1e5,1.5e5,2e5,3e5,1e6,1e7")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--waves-per-block", type=int, default=4)
     ap.add_argument("--donate-min", type=int, default=0)
