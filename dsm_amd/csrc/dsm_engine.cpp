@@ -906,16 +906,18 @@ int upload_records(dsm_engine *e, const std::vector<uint32_t> &s, const std::vec
 
 // One batch's update and candidate refresh from its host delta, enqueued on
 // the engine's stream: reset the per-batch apply counters, apply (device
-// merge/commit, which waits for the commit's status), relabel, refilter the
-// dirty vertices and rebuild their label-sorted copies. No wait at the end.
+// merge/commit, which waits for the commit's status), relabel, rebuild the
+// dirty vertices' label-sorted copies, then refilter the dirty vertices (long
+// lists take their neighbour-label counts from those copies). No wait at the
+// end.
 int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d) {
   int rc;
   HIPCHK(launch_ctl_reset(nullptr, e->actl.p, nullptr, nullptr, 0, e->stream));
   if ((rc = do_apply_delta(e, d))) return rc;
-  HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
   // the dirty list (touched vertices, relabelled ones and their neighbours) is
   // exactly the set whose label-sorted copies changed
   HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, e->dirty_list.p, &e->actl.p->ndirty, e->stream));
+  HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
   return DSM_OK;
 }
 

@@ -472,31 +472,25 @@ __global__ __launch_bounds__(256) void k_filter_light(DevGraph g, FilterTables f
   }
 }
 
-// Dirty vertices with long lists, one wave each; lanes then test bitmaps in parallel.
+// Dirty vertices with long lists (every one has a label-sorted copy, rebuilt
+// for this batch before this kernel runs): the neighbour count of query label
+// index li is the size of label group li, so no list is scanned. One wave per
+// vertex; lanes test the bitmaps in parallel.
 __global__ __launch_bounds__(256) void k_filter_heavy(DevGraph g, FilterTables ft, DirtyArgs d) {
-  __shared__ uint32_t s_cnt[4][kMaxRel];
   __shared__ uint64_t s_pack[4][4];
-  __shared__ uint8_t s_lmap[256];
-  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = ft.lmap[i];
-  __syncthreads();
   const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
   const uint32_t nh = d.actl->nheavy < d.list_cap ? d.actl->nheavy : d.list_cap;
   for (uint32_t h = blockIdx.x * 4 + wv; h < nh; h += gridDim.x * 4) {
     const uint32_t v = d.heavy[h];
     const uint32_t L = g.lab[v];
     const uint32_t dv = g.deg[v];
-    const uint32_t *A = g.pool + g.start[v];
-    if (lane < (uint32_t)kMaxRel) s_cnt[wv][lane] = 0;
-    wave_sync();
-    for (uint32_t j = lane; j < dv; j += 64) {
-      const uint32_t li = s_lmap[g.lab[A[j]]];
-      if (li != 0xFF) atomicAdd(&s_cnt[wv][li], 1u);
-    }
-    wave_sync();
+    const uint32_t *row = g.loff + (size_t)v * (g.nbuckets + 1);
     if (lane < 4) {
       uint64_t x = 0;
       for (int b = 0; b < 8; b++) {
-        const uint32_t y = s_cnt[wv][lane * 8 + b];
+        // groups 0 .. nbuckets-2 are the query labels; the last one, the rest
+        const uint32_t li = lane * 8 + b;
+        const uint32_t y = li + 1 < g.nbuckets ? row[li + 1] - row[li] : 0u;
         x |= (uint64_t)(y < 15 ? y : 15) << (b * 8);
       }
       s_pack[wv][lane] = x;
