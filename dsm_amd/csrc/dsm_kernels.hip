@@ -196,19 +196,38 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
   }
 }
 
+// Work split for the lists of the *_big kernels: a list of at most
+// kBlockList entries is handled by one block (lists are dealt to blocks in
+// turn), a longer one by every thread of the grid. Returns the thread's first
+// index and stride in the list, or false when this block skips it.
+constexpr uint32_t kBlockList = 65536;
+__device__ __forceinline__ bool big_share(uint32_t r, uint32_t len, uint32_t &first, uint32_t &stride) {
+  if (len <= kBlockList) {
+    if (r % gridDim.x != blockIdx.x) return false;
+    first = threadIdx.x;
+    stride = blockDim.x;
+  } else {
+    first = blockIdx.x * blockDim.x + threadIdx.x;
+    stride = gridDim.x * blockDim.x;
+  }
+  return true;
+}
+
 // The merge copies of the lists longer than kApplyWave, each split evenly over
 // every thread of the grid, with the prefix counts k_apply_merge stored.
 __global__ __launch_bounds__(256) void k_apply_merge_big(DevGraph g, BatchDev b, uint32_t *scratch,
                                                          const ApplyCtl *actl) {
   const uint32_t nbig = actl->nmerge_big;
   const uint32_t lane = lane_id();
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
   for (uint32_t r = 0; r < nbig; r++) {
     const uint32_t t = b.big_merge[r];
     const uint32_t x = b.tv[t];
-    const uint32_t *A = g.pool + g.start[x];
     const uint32_t na = g.deg[x];
+    uint32_t tid, nth;
+    if (!big_share(r, na, tid, nth)) continue;
+    // waves of the share: the thread's wave index and the number of waves
+    const uint32_t gw = tid >> 6, nw = nth >> 6;
+    const uint32_t *A = g.pool + g.start[x];
     const uint32_t i0 = b.ioff[t], ni = b.ioff[t + 1] - i0;
     const uint32_t d0 = b.doff[t], nd = b.doff[t + 1] - d0;
     const uint32_t *I = b.ins + i0;
@@ -298,14 +317,14 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_commit(DevGraph g, B
 __global__ __launch_bounds__(256) void k_apply_commit_big(DevGraph g, BatchDev b, const uint32_t *scratch,
                                                           const ApplyCtl *actl) {
   const uint32_t nbig = actl->ncommit_big;
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
   for (uint32_t r = 0; r < nbig; r++) {
     const uint32_t t = b.big_commit[r];
-    const uint32_t x = b.tv[t];
-    const uint32_t dst = g.start[x];
     const uint32_t nd = b.ndeg[t];
+    uint32_t first, stride;
+    if (!big_share(r, nd, first, stride)) continue;
+    const uint32_t dst = g.start[b.tv[t]];
     const unsigned long long so = b.soff[t];
-    for (uint32_t j = tid; j < nd; j += nth) g.pool[dst + j] = scratch[so + j];
+    for (uint32_t j = first; j < nd; j += stride) g.pool[dst + j] = scratch[so + j];
   }
 }
 
@@ -349,15 +368,16 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, BatchDev b, FilterT
 }
 
 // The neighbours of every relabelled vertex of degree > kRelabelWave, marked
-// dirty by the whole grid: each list is split evenly over all threads.
+// dirty (big_share splits the lists).
 __global__ __launch_bounds__(256) void k_relabel_big(DevGraph g, BatchDev b, DirtyArgs d) {
   const uint32_t nbig = d.actl->nrelabel_big;
-  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
   for (uint32_t r = 0; r < nbig; r++) {
     const uint32_t x = b.rl_big[r];
-    const uint32_t *A = g.pool + g.start[x];
     const uint32_t na = g.deg[x];
-    for (uint32_t j = tid; j < na; j += nth) mark_dirty(d, A[j]);
+    uint32_t first, stride;
+    if (!big_share(r, na, first, stride)) continue;
+    const uint32_t *A = g.pool + g.start[x];
+    for (uint32_t j = first; j < na; j += stride) mark_dirty(d, A[j]);
   }
 }
 
