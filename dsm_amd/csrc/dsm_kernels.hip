@@ -1435,6 +1435,12 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
       if (got == 2 && lane == 0) mbytes += list_bytes(S, S.back[L0]);
       init_level(S, a.g, L0, lo, hi);
       uint32_t l = L0;
+      // highest position the leaf set F depends on (-1: none)
+      int fdep = -1;
+      if (n >= 4) {
+        const uint32_t dep = (uint32_t)(S.back[n - 1] | S.inj[n - 1]) & ((1u << (n - 2)) - 1u);
+        fdep = dep ? 31 - __builtin_clz(dep) : -1;
+      }
       uint32_t tick = 0;
       for (;;) {
         if (l == n - 1) {
@@ -1471,8 +1477,10 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
             S.f[l] = w;
             S.fs[l] = ws;
             S.fd[l] = wd;
-            // F depends on positions 0..n-3
-            if (l + 3 <= n) S.fstate = kFStale;
+            // F depends only on the positions of the leaf's backward and
+            // same-label neighbours other than n-2: it stays valid while
+            // deeper positions change
+            if ((int)l <= fdep) S.fstate = kFStale;
           }
           wave_sync();
           l++;
