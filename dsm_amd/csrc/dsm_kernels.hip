@@ -787,7 +787,7 @@ struct WaveHdr {
   uint32_t fstate, fn;       // leaf set F of the current positions 0..n-3 (see leaf_count)
 };
 constexpr uint32_t kWin = 256;   // window entries staged in LDS per intersection
-constexpr uint32_t kFCap = 256;  // largest leaf set F kept in LDS
+constexpr uint32_t kFCap = 2048;  // largest leaf set F kept in LDS
 // states of the leaf set F
 constexpr uint32_t kFStale = 0;  // not computed for the current positions
 constexpr uint32_t kFReady = 1;  // fbuf[0, fn) holds F
