@@ -100,6 +100,8 @@ struct dsm_engine {
   DevBuf<uint32_t> pool, start, deg, cap;
   DevBuf<uint8_t> lab;
   DevBuf<uint32_t> lpool, loff;  // label-sorted copies of long lists
+  DevBuf<uint32_t> huge_list, huge_ctl, huge_hist;  // segmented builds of the longest lists
+  uint32_t huge_segs = 0;
   uint32_t nbuckets = 1;
   unsigned long long pool_cap = 0;
   // filter
@@ -201,6 +203,15 @@ struct dsm_engine {
     t.bm = bm.p;
     t.words = words;
     return t;
+  }
+  HugeSort huge_sort() {
+    HugeSort h;
+    h.list = huge_list.p;
+    h.cap = (uint32_t)(huge_list.n / 2);
+    h.ctl = huge_ctl.p;
+    h.hist = huge_hist.p;
+    h.segs = huge_segs;
+    return h;
   }
   DirtyArgs dirty() {
     DirtyArgs d;
@@ -383,6 +394,21 @@ int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq, const st
 
 // ------------------------------------------------------------- graph set-up
 
+// Segment slots for the segmented label sort: enough for every list of the
+// pool to be split into kHugeSeg-entry segments, plus a margin.
+int size_huge_sort(dsm_engine *e) {
+  const uint32_t segs = (uint32_t)(e->pool_cap / kHugeSeg + 1024);
+  if (segs > e->huge_segs) {
+    HIPCHK(e->huge_hist.alloc((size_t)segs * (kMaxRel + 2)));
+    e->huge_segs = segs;
+  }
+  if (!e->huge_list.p) {
+    HIPCHK(e->huge_list.alloc(2 * 4096));
+    HIPCHK(e->huge_ctl.alloc(2));
+  }
+  return DSM_OK;
+}
+
 int upload_graph(dsm_engine *e, const dsm_graph_desc *g) {
   const uint32_t n = g->num_vertices;
   // sorted, deduplicated CSR
@@ -431,6 +457,7 @@ int upload_graph(dsm_engine *e, const dsm_graph_desc *g) {
     if (e->lmap[l] != 0xFF) e->nbuckets = std::max<uint32_t>(e->nbuckets, e->lmap[l] + 2u);
   HIPCHK(e->loff.alloc((size_t)n * (e->nbuckets + 1)));
   e->pool_cap = pcap;
+  if (int rc = size_huge_sort(e)) return rc;
   HIPCHK(hipMemcpy(e->pool.p, hp.data(), total * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(e->start.p, st.data(), (size_t)n * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(e->deg.p, dg.data(), (size_t)n * 4, hipMemcpyHostToDevice));
@@ -831,8 +858,9 @@ int grow_pool(dsm_engine *e, unsigned long long extra) {
   std::swap(e->start.p, nst.p);
   std::swap(e->start.n, nst.n);
   e->pool_cap = ncap;
+  if (int rc = size_huge_sort(e)) return rc;
   HIPCHK(e->lpool.alloc(ncap));
-  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->stream));
+  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
   // scratch must hold one copy of every list
   if (e->scratch_cap < ncap + (1ull << 20)) {
     e->scratch_cap = ncap + (1ull << 20);
@@ -916,7 +944,7 @@ int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d) {
   if ((rc = do_apply_delta(e, d))) return rc;
   // the dirty list (touched vertices, relabelled ones and their neighbours) is
   // exactly the set whose label-sorted copies changed
-  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, e->dirty_list.p, &e->actl.p->ndirty, e->stream));
+  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, e->dirty_list.p, &e->actl.p->ndirty, e->huge_sort(), e->stream));
   HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
   return DSM_OK;
 }
@@ -971,7 +999,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   e->graph_rounds = e->opt.graph_rounds != 0;
   e->leaf_segments = e->opt.leaf_segments != 0;
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
-  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->stream));
+  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   return DSM_OK;
 }

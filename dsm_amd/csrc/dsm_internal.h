@@ -208,10 +208,23 @@ hipError_t launch_apply_commit_failed(const DevGraph &g, const BatchDev &b, cons
 hipError_t launch_relabel(const DevGraph &g, const BatchDev &b, const FilterTables &ft,
                           const DirtyArgs &d, hipStream_t s);
 hipError_t launch_filter_full(const DevGraph &g, const FilterTables &ft, hipStream_t s);
+// Label-sorted copies of lists longer than kHugeList entries are built in
+// segments of kHugeSeg entries spread over the grid: the vertices are queued
+// here with their first segment slot, and per-segment bucket counts go to hist.
+constexpr uint32_t kHugeList = 65536;
+constexpr uint32_t kHugeSeg = 65536;
+struct HugeSort {
+  uint32_t *list;      // [cap][2]: vertex, first segment slot
+  uint32_t cap;
+  uint32_t *ctl;       // [2]: queued vertices, segment slots taken
+  uint32_t *hist;      // [segs][kMaxRel + 2]
+  uint32_t segs;
+};
+
 // Rebuilds the label-sorted copies of the listed vertices (list + count on the
 // device), or of every vertex when list is null.
 hipError_t launch_lbuild(const DevGraph &g, const uint8_t *lmap, const uint32_t *list, const uint32_t *count,
-                         hipStream_t s);
+                         const HugeSort &hs, hipStream_t s);
 hipError_t launch_filter_dirty(const DevGraph &g, const FilterTables &ft, const DirtyArgs &d,
                                hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);

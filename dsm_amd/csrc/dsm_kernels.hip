@@ -628,10 +628,12 @@ __global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap,
 // stays stable (id order inside each bucket). Control flow around the block
 // barriers depends only on the block's vertex, so it is uniform.
 __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_big(DevGraph g, const uint8_t *lmap,
-                                                                  const uint32_t *list, const uint32_t *count) {
+                                                                  const uint32_t *list, const uint32_t *count,
+                                                                  HugeSort hs) {
   __shared__ uint8_t s_lmap[256];
   __shared__ uint32_t s_cnt[kLbuildWaves][kMaxRel + 2];
   __shared__ uint32_t s_base[kMaxRel + 2];
+  __shared__ uint32_t s_queued;
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = lmap[i];
   __syncthreads();
   const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
@@ -642,6 +644,29 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_big(DevGraph g, co
     const uint32_t v = list ? list[t] : t;
     const uint32_t d = g.deg[v];
     if (d <= kLbuildBlock) continue;
+    if (d > kHugeList) {
+      // queue it for the segmented build when its slots fit, else sort here
+      if (threadIdx.x == 0) {
+        const uint32_t nseg = (d + kHugeSeg - 1) / kHugeSeg;
+        uint32_t ok = 0;
+        const uint32_t i = atomicAdd(&hs.ctl[0], 1u);
+        if (i < hs.cap) {
+          const uint32_t sb = atomicAdd(&hs.ctl[1], nseg);
+          if (sb + nseg <= hs.segs) {
+            hs.list[2 * i] = v;
+            hs.list[2 * i + 1] = sb;
+            ok = 1;
+          } else {
+            hs.list[2 * i] = 0xFFFFFFFFu;  // slot taken but unused
+          }
+        }
+        s_queued = ok;
+      }
+      __syncthreads();
+      const uint32_t queued = s_queued;
+      __syncthreads();
+      if (queued) continue;
+    }
     const uint32_t *A = g.pool + g.start[v];
     uint32_t *row = g.loff + (size_t)v * (nb + 1);
     const uint32_t seg = (d + kLbuildWaves - 1) / kLbuildWaves;
@@ -680,6 +705,126 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_big(DevGraph g, co
     __syncthreads();
     bucket_scatter(g, s_lmap, A, lo, hi, bits, cnt, g.lpool + g.start[v]);
     __syncthreads();
+  }
+}
+
+// Segmented build of the queued lists (longer than kHugeList): segment k of a
+// list is handled by block (slot + k) mod gridDim. Pass 1 stores each
+// segment's bucket counts; pass 2 derives every segment's start inside every
+// bucket from them and scatters the segment (per-wave histograms order the
+// waves inside the segment), so the copy stays stable.
+__device__ __forceinline__ void huge_entry(const HugeSort &hs, uint32_t i, uint32_t &v, uint32_t &sb) {
+  v = hs.list[2 * i];
+  sb = hs.list[2 * i + 1];
+}
+
+__global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_huge_count(DevGraph g, const uint8_t *lmap,
+                                                                         HugeSort hs) {
+  __shared__ uint8_t s_lmap[256];
+  __shared__ uint32_t s_cnt[kLbuildWaves][kMaxRel + 2];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = lmap[i];
+  __syncthreads();
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t nb = g.nbuckets;
+  const uint32_t bits = 32u - (uint32_t)__builtin_clz(nb > 1 ? nb - 1 : 1u);
+  const uint32_t nq = hs.ctl[0] < hs.cap ? hs.ctl[0] : hs.cap;
+  for (uint32_t i = 0; i < nq; i++) {
+    uint32_t v, sb;
+    huge_entry(hs, i, v, sb);
+    if (v == 0xFFFFFFFFu) continue;
+    const uint32_t d = g.deg[v];
+    const uint32_t nseg = (d + kHugeSeg - 1) / kHugeSeg;
+    const uint32_t *A = g.pool + g.start[v];
+    for (uint32_t k = 0; k < nseg; k++) {
+      if ((sb + k) % gridDim.x != blockIdx.x) continue;
+      const uint32_t s0 = k * kHugeSeg, s1 = s0 + kHugeSeg < d ? s0 + kHugeSeg : d;
+      const uint32_t sub = (s1 - s0 + kLbuildWaves - 1) / kLbuildWaves;
+      const uint32_t lo = s0 + wv * sub < s1 ? s0 + wv * sub : s1;
+      const uint32_t hi = lo + sub < s1 ? lo + sub : s1;
+      if (lane < nb) s_cnt[wv][lane] = 0;
+      wave_sync();
+      bucket_count(g, s_lmap, A, lo, hi, bits, s_cnt[wv]);
+      __syncthreads();
+      if (threadIdx.x < nb) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < kLbuildWaves; w++) tot += s_cnt[w][threadIdx.x];
+        hs.hist[(size_t)(sb + k) * (kMaxRel + 2) + threadIdx.x] = tot;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_huge_scatter(DevGraph g, const uint8_t *lmap,
+                                                                           HugeSort hs) {
+  __shared__ uint8_t s_lmap[256];
+  __shared__ uint32_t s_cnt[kLbuildWaves][kMaxRel + 2];
+  __shared__ uint32_t s_tot[kMaxRel + 2], s_before[kMaxRel + 2];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = lmap[i];
+  __syncthreads();
+  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t nb = g.nbuckets;
+  const uint32_t bits = 32u - (uint32_t)__builtin_clz(nb > 1 ? nb - 1 : 1u);
+  const uint32_t nq = hs.ctl[0] < hs.cap ? hs.ctl[0] : hs.cap;
+  for (uint32_t i = 0; i < nq; i++) {
+    uint32_t v, sb;
+    huge_entry(hs, i, v, sb);
+    if (v == 0xFFFFFFFFu) continue;
+    const uint32_t d = g.deg[v];
+    const uint32_t nseg = (d + kHugeSeg - 1) / kHugeSeg;
+    const uint32_t *A = g.pool + g.start[v];
+    uint32_t *row = g.loff + (size_t)v * (nb + 1);
+    for (uint32_t k = 0; k < nseg; k++) {
+      if ((sb + k) % gridDim.x != blockIdx.x) continue;
+      // bucket totals over all segments, and this segment's share before it
+      if (threadIdx.x < nb) {
+        uint32_t tot = 0, before = 0;
+        for (uint32_t k2 = 0; k2 < nseg; k2++) {
+          const uint32_t c = hs.hist[(size_t)(sb + k2) * (kMaxRel + 2) + threadIdx.x];
+          tot += c;
+          if (k2 < k) before += c;
+        }
+        s_tot[threadIdx.x] = tot;
+        s_before[threadIdx.x] = before;
+      }
+      const uint32_t s0 = k * kHugeSeg, s1 = s0 + kHugeSeg < d ? s0 + kHugeSeg : d;
+      const uint32_t sub = (s1 - s0 + kLbuildWaves - 1) / kLbuildWaves;
+      const uint32_t lo = s0 + wv * sub < s1 ? s0 + wv * sub : s1;
+      const uint32_t hi = lo + sub < s1 ? lo + sub : s1;
+      if (lane < nb) s_cnt[wv][lane] = 0;
+      wave_sync();
+      bucket_count(g, s_lmap, A, lo, hi, bits, s_cnt[wv]);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < nb; b++) {
+          const uint32_t c = s_tot[b];
+          s_tot[b] = run;  // bucket start
+          if (k == 0) row[b] = run;
+          run += c;
+        }
+        if (k == 0) row[nb] = run;
+      }
+      __syncthreads();
+      if (threadIdx.x < nb) {
+        uint32_t run = s_tot[threadIdx.x] + s_before[threadIdx.x];
+        for (uint32_t w = 0; w < kLbuildWaves; w++) {
+          const uint32_t c = s_cnt[w][threadIdx.x];
+          s_cnt[w][threadIdx.x] = run;
+          run += c;
+        }
+      }
+      __syncthreads();
+      bucket_scatter(g, s_lmap, A, lo, hi, bits, s_cnt[wv], g.lpool + g.start[v]);
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void k_huge_reset(HugeSort hs) {
+  if (threadIdx.x == 0) {
+    hs.ctl[0] = 0;
+    hs.ctl[1] = 0;
   }
 }
 
@@ -1689,12 +1834,18 @@ hipError_t launch_filter_dirty(const DevGraph &g, const FilterTables &ft, const 
 }
 
 hipError_t launch_lbuild(const DevGraph &g, const uint8_t *lmap, const uint32_t *list, const uint32_t *count,
-                         hipStream_t s) {
+                         const HugeSort &hs, hipStream_t s) {
   const uint32_t blocks = list ? 2048u : ((g.n + 3) / 4 < 8192 ? (g.n + 3) / 4 : 8192u);
+  hipLaunchKernelGGL(k_huge_reset, dim3(1), dim3(64), 0, s, hs);
   hipLaunchKernelGGL(k_lbuild, dim3(blocks), dim3(256), 0, s, g, lmap, list, count);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_lbuild_big, dim3(list ? 256u : 1024u), dim3(kLbuildWaves * 64), 0, s, g, lmap, list, count);
+  hipLaunchKernelGGL(k_lbuild_big, dim3(list ? 256u : 1024u), dim3(kLbuildWaves * 64), 0, s, g, lmap, list, count,
+                     hs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lbuild_huge_count, dim3(512), dim3(kLbuildWaves * 64), 0, s, g, lmap, hs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lbuild_huge_scatter, dim3(512), dim3(kLbuildWaves * 64), 0, s, g, lmap, hs);
   return hipGetLastError();
 }
 

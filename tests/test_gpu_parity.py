@@ -428,3 +428,34 @@ def test_label_sorted_copy_of_big_hub():
     nb = rng.choice(np.setdiff1d(np.arange(1, n), hub_nbrs), k, replace=False).astype(np.uint32)
     batches.append((np.ones(k, np.uint8), np.zeros(k, np.uint32), nb, rng.integers(0, nl, k).astype(np.uint32)))
     compare_stream(n, labels, eu, ev, queries, batches, relabel=True, check_graph=True)
+
+
+def test_label_sorted_copy_of_huge_hub():
+    """A hub of degree 140,000 (its label-sorted copy is built in 65,536-entry
+    segments spread over the grid), relabelled neighbours and inserts at the
+    hub: counts, graph and bitmaps against the oracle."""
+    say("case: hub of degree 140,000")
+    rng = np.random.default_rng(1515)
+    n, nl = 150000, 4
+    labels = rng.integers(0, nl, n).astype(np.uint32)
+    hub_nbrs = rng.choice(np.arange(1, n), 140000, replace=False)
+    a = rng.integers(1, n, 40000)
+    b = rng.integers(1, n, 40000)
+    keep = a != b
+    e = np.concatenate([np.stack([np.zeros(len(hub_nbrs), np.int64), hub_nbrs], 1),
+                        np.stack([np.minimum(a[keep], b[keep]), np.maximum(a[keep], b[keep])], 1)])
+    e = np.unique(e, axis=0).astype(np.uint32)
+    eu, ev = e[:, 0].copy(), e[:, 1].copy()
+    x = int(labels[0])
+    # one label everywhere: inserts relabel both endpoints alike, so only
+    # same-label query edges can be seeded by them
+    queries = [qtuple("triangle", [x, x, x]), qtuple("square", [x, x, x, x])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batches = []
+    for _ in range(2):
+        add, src, dst, lab = conflict_free_batch(n, 400, present, rng)
+        batches.append((add, src, dst, rng.integers(0, nl, len(src)).astype(np.uint32)))
+    k = 100
+    nb = rng.choice(np.setdiff1d(np.arange(1, n), hub_nbrs), k, replace=False).astype(np.uint32)
+    batches.append((np.ones(k, np.uint8), np.zeros(k, np.uint32), nb, rng.integers(0, nl, k).astype(np.uint32)))
+    compare_stream(n, labels, eu, ev, queries, batches, relabel=True, check_graph=True)
