@@ -211,6 +211,9 @@ struct dsm_engine {
     h.ctl = huge_ctl.p;
     h.hist = huge_hist.p;
     h.segs = huge_segs;
+    // the merge scratch (at least one entry per pool entry) is idle while the
+    // label-sorted copies are rebuilt
+    h.bucket_bytes = scratch_cap >= pool_cap ? reinterpret_cast<uint8_t *>(scratch.p) : nullptr;
     return h;
   }
   DirtyArgs dirty() {
@@ -860,12 +863,12 @@ int grow_pool(dsm_engine *e, unsigned long long extra) {
   e->pool_cap = ncap;
   if (int rc = size_huge_sort(e)) return rc;
   HIPCHK(e->lpool.alloc(ncap));
-  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
   // scratch must hold one copy of every list
   if (e->scratch_cap < ncap + (1ull << 20)) {
     e->scratch_cap = ncap + (1ull << 20);
     HIPCHK(e->scratch.alloc(e->scratch_cap));
   }
+  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
   ApplyCtl a = *e->h_actl.p;
   a.pool_used = total;
   a.overflow = 0;

@@ -219,6 +219,10 @@ struct HugeSort {
   uint32_t *ctl;       // [2]: queued vertices, segment slots taken
   uint32_t *hist;      // [segs][kMaxRel + 2]
   uint32_t segs;
+  // one byte per pool entry (indexed like the pool): the bucket of each entry,
+  // written by the counting pass so that the scatter pass reads it back
+  // instead of gathering the neighbour's label again (null: gather twice)
+  uint8_t *bucket_bytes;
 };
 
 // Rebuilds the label-sorted copies of the listed vertices (list + count on the

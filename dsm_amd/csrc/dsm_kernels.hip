@@ -546,12 +546,14 @@ __device__ __forceinline__ uint32_t bucket_of(const DevGraph &g, const uint8_t *
 // Adds the bucket histogram of A[lo, hi) to cnt (per-wave LDS counters; the
 // lowest lane of each peer group adds the group's size).
 __device__ __forceinline__ void bucket_count(const DevGraph &g, const uint8_t *s_lmap, const uint32_t *A,
-                                             uint32_t lo, uint32_t hi, uint32_t bits, uint32_t *cnt) {
+                                             uint32_t lo, uint32_t hi, uint32_t bits, uint32_t *cnt,
+                                             const uint8_t *bk_in, uint8_t *bk_out) {
   const uint32_t lane = lane_id();
   for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
     const uint32_t j = c0 + lane;
     const bool valid = j < hi;
-    const uint32_t b = valid ? bucket_of(g, s_lmap, A[j]) : 0u;
+    const uint32_t b = valid ? (bk_in ? (uint32_t)bk_in[j] : bucket_of(g, s_lmap, A[j])) : 0u;
+    if (valid && bk_out) bk_out[j] = (uint8_t)b;
     const uint64_t pm = bucket_peers(b, valid, bits);
     if (valid && mbcnt(pm) == 0) cnt[b] += (uint32_t)__popcll(pm);
     wave_sync();
@@ -562,13 +564,13 @@ __device__ __forceinline__ void bucket_count(const DevGraph &g, const uint8_t *s
 // bucket b; id order is kept inside each bucket.
 __device__ __forceinline__ void bucket_scatter(const DevGraph &g, const uint8_t *s_lmap, const uint32_t *A,
                                                uint32_t lo, uint32_t hi, uint32_t bits, uint32_t *cnt,
-                                               uint32_t *out) {
+                                               uint32_t *out, const uint8_t *bk_in) {
   const uint32_t lane = lane_id();
   for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
     const uint32_t j = c0 + lane;
     const bool valid = j < hi;
     const uint32_t x = valid ? A[j] : 0u;
-    const uint32_t b = valid ? bucket_of(g, s_lmap, x) : 0u;
+    const uint32_t b = valid ? (bk_in ? (uint32_t)bk_in[j] : bucket_of(g, s_lmap, x)) : 0u;
     const uint64_t pm = bucket_peers(b, valid, bits);
     const uint32_t rank = mbcnt(pm);
     const uint32_t base = valid ? cnt[b] : 0u;
@@ -588,7 +590,7 @@ constexpr uint32_t kLbuildWaves = 16;    // waves of that block
 // histogram, offsets, then a stable scatter of the id-sorted list into lpool
 // bucket by bucket. Longer lists are left to k_lbuild_big.
 __global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap, const uint32_t *list,
-                                                const uint32_t *count) {
+                                                const uint32_t *count, uint8_t *bucket_bytes) {
   __shared__ uint8_t s_lmap[256];
   __shared__ uint32_t s_cnt[4][kMaxRel + 2];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = lmap[i];
@@ -606,7 +608,8 @@ __global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap,
     uint32_t *row = g.loff + (size_t)v * (nb + 1);
     if (lane < nb) cnt[lane] = 0;
     wave_sync();
-    bucket_count(g, s_lmap, A, 0, d, bits, cnt);
+    uint8_t *bk = bucket_bytes ? bucket_bytes + g.start[v] : nullptr;
+    bucket_count(g, s_lmap, A, 0, d, bits, cnt, nullptr, bk);
     if (lane == 0) {
       uint32_t run = 0;
       for (uint32_t k = 0; k < nb; k++) {
@@ -618,7 +621,7 @@ __global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap,
       row[nb] = run;
     }
     wave_sync();
-    bucket_scatter(g, s_lmap, A, 0, d, bits, cnt, g.lpool + g.start[v]);
+    bucket_scatter(g, s_lmap, A, 0, d, bits, cnt, g.lpool + g.start[v], bk);
   }
 }
 
@@ -673,9 +676,10 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_big(DevGraph g, co
     const uint32_t lo = wv * seg < d ? wv * seg : d;
     const uint32_t hi = lo + seg < d ? lo + seg : d;
     uint32_t *cnt = s_cnt[wv];
+    uint8_t *bk = hs.bucket_bytes ? hs.bucket_bytes + g.start[v] : nullptr;
     if (lane < nb) cnt[lane] = 0;
     wave_sync();
-    bucket_count(g, s_lmap, A, lo, hi, bits, cnt);
+    bucket_count(g, s_lmap, A, lo, hi, bits, cnt, nullptr, bk);
     __syncthreads();
     if (threadIdx.x < nb) {
       uint32_t tot = 0;
@@ -703,7 +707,7 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_big(DevGraph g, co
       }
     }
     __syncthreads();
-    bucket_scatter(g, s_lmap, A, lo, hi, bits, cnt, g.lpool + g.start[v]);
+    bucket_scatter(g, s_lmap, A, lo, hi, bits, cnt, g.lpool + g.start[v], bk);
     __syncthreads();
   }
 }
@@ -741,9 +745,10 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_huge_count(DevGrap
       const uint32_t sub = (s1 - s0 + kLbuildWaves - 1) / kLbuildWaves;
       const uint32_t lo = s0 + wv * sub < s1 ? s0 + wv * sub : s1;
       const uint32_t hi = lo + sub < s1 ? lo + sub : s1;
+      uint8_t *bk = hs.bucket_bytes ? hs.bucket_bytes + g.start[v] : nullptr;
       if (lane < nb) s_cnt[wv][lane] = 0;
       wave_sync();
-      bucket_count(g, s_lmap, A, lo, hi, bits, s_cnt[wv]);
+      bucket_count(g, s_lmap, A, lo, hi, bits, s_cnt[wv], nullptr, bk);
       __syncthreads();
       if (threadIdx.x < nb) {
         uint32_t tot = 0;
@@ -791,9 +796,10 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_huge_scatter(DevGr
       const uint32_t sub = (s1 - s0 + kLbuildWaves - 1) / kLbuildWaves;
       const uint32_t lo = s0 + wv * sub < s1 ? s0 + wv * sub : s1;
       const uint32_t hi = lo + sub < s1 ? lo + sub : s1;
+      const uint8_t *bk = hs.bucket_bytes ? hs.bucket_bytes + g.start[v] : nullptr;
       if (lane < nb) s_cnt[wv][lane] = 0;
       wave_sync();
-      bucket_count(g, s_lmap, A, lo, hi, bits, s_cnt[wv]);
+      bucket_count(g, s_lmap, A, lo, hi, bits, s_cnt[wv], bk, nullptr);
       __syncthreads();
       if (threadIdx.x == 0) {
         uint32_t run = 0;
@@ -815,7 +821,7 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_huge_scatter(DevGr
         }
       }
       __syncthreads();
-      bucket_scatter(g, s_lmap, A, lo, hi, bits, s_cnt[wv], g.lpool + g.start[v]);
+      bucket_scatter(g, s_lmap, A, lo, hi, bits, s_cnt[wv], g.lpool + g.start[v], bk);
       __syncthreads();
     }
   }
@@ -1837,7 +1843,7 @@ hipError_t launch_lbuild(const DevGraph &g, const uint8_t *lmap, const uint32_t 
                          const HugeSort &hs, hipStream_t s) {
   const uint32_t blocks = list ? 2048u : ((g.n + 3) / 4 < 8192 ? (g.n + 3) / 4 : 8192u);
   hipLaunchKernelGGL(k_huge_reset, dim3(1), dim3(64), 0, s, hs);
-  hipLaunchKernelGGL(k_lbuild, dim3(blocks), dim3(256), 0, s, g, lmap, list, count);
+  hipLaunchKernelGGL(k_lbuild, dim3(blocks), dim3(256), 0, s, g, lmap, list, count, hs.bucket_bytes);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_lbuild_big, dim3(list ? 256u : 1024u), dim3(kLbuildWaves * 64), 0, s, g, lmap, list, count,
