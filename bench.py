@@ -449,9 +449,16 @@ def main():
         cpu_n = run_cpu_threads(w, args.warmup, threads)
         cpu_1 = run_cpu(w, args.warmup, args.cpu_budget)
         log("cpu: %d threads %.1f updates/s; 1 thread %.1f updates/s" % (threads, cpu_n["value"], cpu_1["value"]))
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+        except OSError:
+            pass
         out["cpu_baseline"] = {"value": round(cpu_n["value"], 2), "unit": "edge updates/s", "cores": threads,
                                "kind": "port", "sample": cpu_n["sample"],
-                               "one_core": {"value": round(cpu_1["value"], 2), "sample": cpu_1["sample"]}}
+                               "one_core": {"value": round(cpu_1["value"], 2), "sample": cpu_1["sample"]},
+                               "host": {"nproc": os.cpu_count(), "cpu_model": model}}
     print(json.dumps(out), flush=True)
 
 

@@ -1,20 +1,24 @@
 // HIP kernels of the MI355X continuous subgraph matching engine (gfx950,
 // wave64). Integer work only; no MFMA.
 //
-//   k_apply_merge / k_apply_commit  in-place batch update of the slack CSR
+//   k_apply_merge / k_apply_commit (+ _big for hub lists)
+//                                   in-place batch update of the slack CSR
 //                                   (the reference rebuilds the whole graph on
 //                                   the host, cpuGraph.cpp:521-607, and
 //                                   re-uploads it, io.cu:279-302)
-//   k_relabel                       label changes of the update rule
+//   k_relabel (+ _big)              label changes of the update rule
 //                                   (cpuGraph.cpp:564) + dirty marking
+//   k_lbuild / k_lbuild_big / k_lbuild_huge_*
+//                                   label-sorted copies of lists longer than
+//                                   64 entries, rebuilt for the dirty set
 //   k_filter_full / k_filter_light / k_filter_heavy
 //                                   LDF/NLF candidate bitmaps
 //                                   (filter.cu:394-566, compact filter.cu:1692)
 //   k_seed                          device edge seeding (main.cu:268-329;
 //                                   replaces the host ResTable and join.cu)
 //   k_dfs                           persistent wave64 DFS with per-wave LDS
-//                                   stacks and work donation (gpu_match.cu:
-//                                   96-876 restated for CDNA)
+//                                   stacks, work donation and shared leaf sets
+//                                   (gpu_match.cu:96-876 restated for CDNA)
 #include "dsm_internal.h"
 
 namespace dsm {
