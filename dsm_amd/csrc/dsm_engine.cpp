@@ -136,7 +136,7 @@ struct dsm_engine {
   bool tuples_host = false;
   unsigned long long tuple_cap = 0;
   uint32_t dfs_blocks = 0, dfs_wpb = 4, max_qn = 2;
-  uint32_t donate_min = 256, lane_work = 16, donate_every = 32, donate_min_n2 = 64;
+  uint32_t donate_min = 256, lane_work = 16, donate_every = 8, donate_min_n2 = 64;
   bool leaf_sets = true;
   bool graph_rounds = true;
   bool leaf_segments = true;
@@ -996,7 +996,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   }
   e->donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
   e->lane_work = e->opt.lane_work ? e->opt.lane_work : 16;
-  e->donate_every = e->opt.donate_every ? e->opt.donate_every : 32;
+  e->donate_every = e->opt.donate_every ? e->opt.donate_every : 8;
   e->donate_min_n2 = e->opt.donate_min_n2 ? e->opt.donate_min_n2 : 64;
   e->leaf_sets = e->opt.leaf_sets != 0;
   e->graph_rounds = e->opt.graph_rounds != 0;
@@ -1240,6 +1240,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
   struct Cfg {
     uint32_t wpb, bpc, dmin, lw;
     bool ls, seg;
+    uint32_t dev;
   };
   auto max_bpc = [&](uint32_t wpb) { return (uint32_t)std::max(1, dfs_max_blocks_per_cu(wpb, e->max_qn)); };
   // times one sample round under c and checks its counts against the first
@@ -1252,6 +1253,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     e->lane_work = c.lw;
     e->leaf_sets = c.ls;
     e->leaf_segments = c.seg;
+    e->donate_every = c.dev;
     HIPCHK(launch_ctl_reset(nullptr, nullptr, e->tuple_count.p, e->counts.p, nq * 2, e->stream));
     int rc2 = run_round(e, rb.p, rb.p + k, k, 0, e->ev[9], e->ev[10], e->ev[11]);
     if (rc2) return rc2;
@@ -1267,7 +1269,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     return DSM_OK;
   };
   Cfg best{e->dfs_wpb, e->dfs_blocks / (uint32_t)e->num_cu, e->donate_min, e->lane_work, e->leaf_sets,
-           e->leaf_segments};
+           e->leaf_segments, e->donate_every};
   // warm-up: clocks and caches settle before anything is compared
   float best_ms = 1e30f;
   for (int r = 0; r < 4; r++) {
@@ -1320,13 +1322,20 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
       if ((rc = consider(c))) return rc;
     }
   }
-  // 3. donation threshold
+  // 3. donation threshold and how often busy waves look for idle ones
   {
     const Cfg base = best;
     for (uint32_t v : {64u, 128u, 256u, 512u, 1024u}) {
       if (v == base.dmin) continue;
       Cfg c = base;
       c.dmin = v;
+      if ((rc = consider(c))) return rc;
+    }
+    const Cfg base2 = best;
+    for (uint32_t v : {4u, 8u, 16u, 32u}) {
+      if (v == base2.dev) continue;
+      Cfg c = base2;
+      c.dev = v;
       if ((rc = consider(c))) return rc;
     }
   }
@@ -1362,6 +1371,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     rep->lane_work = best.lw;
     rep->leaf_sets = best.ls ? 1 : 0;
     rep->leaf_segments = best.seg ? 1 : 0;
+    rep->donate_every = best.dev;
     rep->trials = trials;
     rep->ms_initial = ms0;
     rep->ms_best = best_ms;

@@ -72,7 +72,8 @@ class Options(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("donate_min", C.c_uint32),
-                ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("leaf_segments", C.c_int), ("trials", C.c_uint32),
+                ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("leaf_segments", C.c_int),
+                ("donate_every", C.c_uint32), ("trials", C.c_uint32),
                 ("ms_initial", C.c_float), ("ms_best", C.c_float)]
 
     def as_dict(self):

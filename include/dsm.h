@@ -84,7 +84,7 @@ typedef struct {
   int leaf_sets;             /* 1: one level above the leaf, candidates count their
                                 leaf images against a leaf set shared by the parent;
                                 0: one descent per candidate */
-  uint32_t donate_every;     /* scan steps between a busy wave's checks for idle waves (0 = 32) */
+  uint32_t donate_every;     /* scan steps between a busy wave's checks for idle waves (0 = 8) */
   uint32_t donate_min_n2;    /* smallest range one level above the leaf that is split
                                 off to an idle wave (0 = 64) */
   int leaf_segments;         /* 1: with leaf sets, the lanes' leaf lists are scanned as
@@ -125,6 +125,7 @@ typedef struct {
 typedef struct {
   uint32_t waves_per_block, blocks_per_cu, donate_min, lane_work;
   int leaf_sets, leaf_segments;
+  uint32_t donate_every;
   uint32_t trials;        /* configurations timed */
   float ms_initial, ms_best;
 } dsm_tune_report;
