@@ -340,6 +340,7 @@ def main():
     ap.add_argument("--donate-every", type=int, default=0)
     ap.add_argument("--donate-min-n2", type=int, default=0)
     ap.add_argument("--leaf-segments", type=int, default=1)
+    ap.add_argument("--poll-max", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run one D2D copy of this many bytes")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -357,7 +358,8 @@ def main():
     engine_kw = {"waves_per_block": args.waves_per_block, "donate_min": args.donate_min,
                  "blocks_per_cu": args.blocks_per_cu, "lane_work": args.lane_work,
                  "leaf_sets": bool(args.leaf_sets), "donate_every": args.donate_every,
-                 "donate_min_n2": args.donate_min_n2, "leaf_segments": bool(args.leaf_segments)}
+                 "donate_min_n2": args.donate_min_n2, "leaf_segments": bool(args.leaf_segments),
+                 "poll_max": args.poll_max}
 
     batch = int(w.bstart[1] - w.bstart[0])
     bat = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune)

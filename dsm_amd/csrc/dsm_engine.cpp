@@ -140,6 +140,7 @@ struct dsm_engine {
   bool leaf_sets = true;
   bool graph_rounds = true;
   bool leaf_segments = true;
+  uint32_t poll_max = 16;
   // per-wave task time of the last DFS launch (balance statistics)
   DevBuf<unsigned long long> wave_busy;
   PinnedBuf<unsigned long long> h_wave_busy;
@@ -586,6 +587,7 @@ DfsArgs dfs_args(dsm_engine *e) {
   da.donate_every = e->donate_every;
   da.donate_min_n2 = e->donate_min_n2;
   da.leaf_segments = e->leaf_segments ? 1u : 0u;
+  da.poll_max = e->poll_max;
   da.leaf_sets = e->leaf_sets ? 1u : 0u;
   da.nq = (uint32_t)e->queries.size();
   da.watchdog = (unsigned long long)e->wall_khz * e->watchdog_ms;
@@ -1001,6 +1003,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   e->leaf_sets = e->opt.leaf_sets != 0;
   e->graph_rounds = e->opt.graph_rounds != 0;
   e->leaf_segments = e->opt.leaf_segments != 0;
+  e->poll_max = e->opt.poll_max ? std::min<uint32_t>(e->opt.poll_max, 16u) : 16u;
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
   HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));

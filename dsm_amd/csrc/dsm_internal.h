@@ -156,6 +156,7 @@ struct DfsArgs {
   uint32_t donate_every;         // chunks between checks for waiting waves
   uint32_t donate_min_n2;        // smallest range split off one level above the leaf
   uint32_t leaf_segments;        // leaf lists scanned as one segmented array
+  uint32_t poll_max;             // longest sleep between polls of a waiting wave (s_sleep 8 units)
   uint32_t leaf_sets;            // fused leaf level on/off
   uint32_t nq;                   // queries (count layout)
   const uint32_t *seed_tag;      // per seed: count slot << 1 | sign

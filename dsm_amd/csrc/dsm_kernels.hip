@@ -1528,7 +1528,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
         expired = bcast(expired);
         if (left <= 0 || ab || expired) break;
       }
-      const uint32_t reps = naps < 4 ? (1u << naps) : 16u;
+      const uint32_t reps = (1u << (naps < 4 ? naps : 4u)) < a.poll_max ? (1u << (naps < 4 ? naps : 4u)) : a.poll_max;
       for (uint32_t r = 0; r < reps; r++) __builtin_amdgcn_s_sleep(8);
       naps++;
       continue;

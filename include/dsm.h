@@ -87,6 +87,8 @@ typedef struct {
   uint32_t donate_every;     /* scan steps between a busy wave's checks for idle waves (0 = 8) */
   uint32_t donate_min_n2;    /* smallest range one level above the leaf that is split
                                 off to an idle wave (0 = 64) */
+  uint32_t poll_max;         /* a wave waiting for donated work sleeps at most this many
+                                8-unit s_sleeps between polls (1..16; 0 = 16) */
   int leaf_segments;         /* 1: with leaf sets, the lanes' leaf lists are scanned as
                                 one segmented array; 0: each lane scans its own */
   int graph_rounds;          /* 1: each DFS round (control reset, seeding, DFS, status
