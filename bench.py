@@ -71,7 +71,28 @@ def tune_sample(w, nb=4):
     return tuple(np.concatenate([p[i] for p in parts]) for i in range(4))
 
 
-def run_batched(w, warmup, steps, engine_kw, autotune=True):
+def _agg_add(agg, st):
+    agg["ms_dfs"] += st["ms_dfs_pos"] + st["ms_dfs_neg"]
+    agg["launches"] += st["dfs_launches"]
+    agg["entries"] += st["dfs_entries"]
+    agg["loads"] += st["dfs_search_loads"]
+    agg["merge_bytes"] += st["dfs_merge_bytes"]
+    agg["busy_ms"] += st["dfs_busy_ms"]
+    agg["waves"] += st["dfs_waves"]
+    agg["imbalance"] += st["dfs_block_imbalance"]
+    agg["ms_apply"] += st["ms_apply"]
+    agg["ms_filter"] += st["ms_filter"]
+    agg["ms_seed"] += st["ms_seed"]
+    agg["seeds"] += st["seeds_pos"] + st["seeds_neg"]
+    agg["donated"] += st["tasks_donated"]
+
+
+def run_batched(w, warmup, steps, engine_kw, autotune=True, pipelined=True):
+    """One engine for all queries. Warm-up batches go through dsm_process_batch;
+    the timed ones through dsm_process_batches (pipelined: the positives of a
+    batch share a launch with the next batch's negatives) or, with
+    pipelined=False, dsm_process_batch one at a time. A pipelined batch's
+    latency is the time between its counts and the previous batch's."""
     from dsm_amd import engine as E
     t0 = time.perf_counter()
     eng = E.Engine((w.n, w.labels, w.eu, w.ev), qtuples(w), **engine_kw)
@@ -83,43 +104,46 @@ def run_batched(w, warmup, steps, engine_kw, autotune=True):
         log("autotune %.2f s: %s" % (time.perf_counter() - t0, tuned))
     lat, recs = [], 0
     agg = {"ms_dfs": 0.0, "launches": 0, "entries": 0, "loads": 0, "merge_bytes": 0, "busy_ms": 0.0, "waves": 0,
-           "imbalance": 0.0, "ms_apply": 0.0,
-           "ms_filter": 0.0,
-           "ms_seed": 0.0, "seeds": 0, "donated": 0}
+           "imbalance": 0.0, "ms_apply": 0.0, "ms_filter": 0.0, "ms_seed": 0.0, "seeds": 0, "donated": 0}
     pos_tot = neg_tot = 0
     per_batch = []
     try:
-        for b in range(warmup + steps):
-            batch = w.batch(b)
+        for b in range(warmup):
             t = time.perf_counter()
-            pos, neg = eng.process_batch(*batch)
-            dt = time.perf_counter() - t
-            st = eng.stats()
+            pos, neg = eng.process_batch(*w.batch(b))
             per_batch.append((pos.copy(), neg.copy()))
-            if b < warmup:
-                log("warmup batch %d: %.2f ms" % (b, dt * 1e3))
-                continue
-            lat.append(dt)
-            recs += len(batch[1])
-            pos_tot += int(pos.sum())
-            neg_tot += int(neg.sum())
-            agg["ms_dfs"] += st["ms_dfs_pos"] + st["ms_dfs_neg"]
-            agg["launches"] += st["dfs_launches"]
-            agg["entries"] += st["dfs_entries"]
-            agg["loads"] += st["dfs_search_loads"]
-            agg["merge_bytes"] += st["dfs_merge_bytes"]
-            agg["busy_ms"] += st["dfs_busy_ms"]
-            agg["waves"] += st["dfs_waves"]
-            agg["imbalance"] += st["dfs_block_imbalance"]
-            agg["ms_apply"] += st["ms_apply"]
-            agg["ms_filter"] += st["ms_filter"]
-            agg["ms_seed"] += st["ms_seed"]
-            agg["seeds"] += st["seeds_pos"] + st["seeds_neg"]
-            agg["donated"] += st["tasks_donated"]
-            log("batch %d: %.2f ms  pos %d neg %d  dfs %.2f ms (%d launches) apply %.2f filter %.2f seeds %d "
-                "donated %d" % (b, dt * 1e3, int(pos.sum()), int(neg.sum()), st["ms_dfs_pos"] + st["ms_dfs_neg"],
-                                st["dfs_launches"], st["ms_apply"], st["ms_filter"],
-                                st["seeds_pos"] + st["seeds_neg"], st["tasks_donated"]))
+            log("warmup batch %d: %.2f ms" % (b, (time.perf_counter() - t) * 1e3))
+        timed = [w.batch(b) for b in range(warmup, warmup + steps)]
+        recs = sum(len(x[1]) for x in timed)
+        if pipelined:
+            t0m = time.monotonic()
+            pos, neg, done = eng.process_batches(timed)
+            prev = t0m
+            for b in range(steps):
+                lat.append(done[b] - prev)
+                prev = done[b]
+                per_batch.append((pos[b].copy(), neg[b].copy()))
+                pos_tot += int(pos[b].sum())
+                neg_tot += int(neg[b].sum())
+            st = eng.stats()
+            _agg_add(agg, st)
+            log("pipelined %d batches: %.2f ms, dfs %.2f ms (%d launches), apply %.2f ms, donated %d"
+                % (steps, (done[-1] - t0m) * 1e3, st["ms_dfs_pos"], st["dfs_launches"], st["ms_apply"],
+                   st["tasks_donated"]))
+        else:
+            for i, batch in enumerate(timed):
+                t = time.perf_counter()
+                pos, neg = eng.process_batch(*batch)
+                dt = time.perf_counter() - t
+                st = eng.stats()
+                per_batch.append((pos.copy(), neg.copy()))
+                lat.append(dt)
+                pos_tot += int(pos.sum())
+                neg_tot += int(neg.sum())
+                _agg_add(agg, st)
+                log("batch %d: %.2f ms  pos %d neg %d  dfs %.2f ms (%d launches) apply %.2f seeds %d donated %d"
+                    % (warmup + i, dt * 1e3, int(pos.sum()), int(neg.sum()), st["ms_dfs_pos"] + st["ms_dfs_neg"],
+                       st["dfs_launches"], st["ms_apply"], st["seeds_pos"] + st["seeds_neg"], st["tasks_donated"]))
     finally:
         eng.close()
     return {"lat": lat, "records": recs, "pos": pos_tot, "neg": neg_tot, "agg": agg, "per_batch": per_batch,
@@ -362,7 +386,10 @@ def main():
                  "poll_max": args.poll_max}
 
     batch = int(w.bstart[1] - w.bstart[0])
-    bat = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune)
+    one = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune, pipelined=False)
+    bat = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune, pipelined=True)
+    for b, ((p1, n1), (p2, n2)) in enumerate(zip(one["per_batch"], bat["per_batch"])):
+        assert np.array_equal(p1, p2) and np.array_equal(n1, n2), "pipelined and one-by-one differ at batch %d" % b
     total_s = sum(bat["lat"])
     value = bat["records"] / total_s
     ms_per_step = total_s / len(bat["lat"]) * 1e3
@@ -400,9 +427,14 @@ def main():
            "config": {"workload": "%s: %s, %d-record batches, %d standing queries, batched (one persistent "
                                   "launch per graph state)" % (w.name, GRAPH_TEXT.get(w.name, ""), batch,
                                                                 len(w.queries)),
-                      "batch": batch, "queries": len(w.queries), "mode": "batched",
+                      "batch": batch, "queries": len(w.queries),
+                      "mode": "batched, pipelined (a batch's positives share a launch with the next batch's negatives)",
                       "parallelism": "single GPU", "tuned": bat["tuned"]},
            "p50_batch_ms": round(pct(bat["lat"], 50) * 1e3, 3), "p99_batch_ms": round(pct(bat["lat"], 99) * 1e3, 3),
+           "one_batch_per_call": {"value": round(one["records"] / sum(one["lat"]), 1),
+                                  "p50_batch_ms": round(pct(one["lat"], 50) * 1e3, 3),
+                                  "p99_batch_ms": round(pct(one["lat"], 99) * 1e3, 3),
+                                  "ms_dfs_per_batch": round(one["agg"]["ms_dfs"] / len(one["lat"]), 3)},
            "matches_per_s": round((bat["pos"] + bat["neg"]) / total_s, 1),
            "stages_ms_per_batch": {k: round(agg[k] / len(bat["lat"]), 3) for k in ("ms_apply", "ms_filter",
                                                                                "ms_seed", "ms_dfs")},

@@ -139,6 +139,8 @@ struct dsm_engine {
   uint32_t donate_min = 256, lane_work = 16, donate_every = 8, donate_min_n2 = 64;
   bool leaf_sets = true;
   bool graph_rounds = true;
+  PinnedBuf<uint32_t> pipe_h_rec;  // records of a pipelined round (dsm_process_batches)
+  DevBuf<uint32_t> pipe_d_rec;
   bool leaf_segments = true;
   uint32_t poll_max = 16;
   // per-wave task time of the last DFS launch (balance statistics)
@@ -1413,15 +1415,37 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-struct ServerReq {
+// One update batch in the rolling pipeline (served stream and
+// dsm_process_batches): its records, split into positive and negative seed
+// lists, its count slot and, once published, its counts.
+struct PipeItem {
+  const dsm_change *rec = nullptr;
+  uint32_t nrec = 0;
+  std::vector<uint32_t> ps, pd, ns, nd;
+  uint32_t slot = 0;
+  std::vector<uint64_t> pos, neg;
+};
+
+void pipe_split(const dsm_engine *e, PipeItem &it) {
+  for (uint32_t i = 0; i < it.nrec; i++) {
+    const dsm_change &c = it.rec[i];
+    if (c.add) {
+      it.ps.push_back(c.src);
+      it.pd.push_back(c.dst);
+    } else if (e->opt.count_negatives) {
+      it.ns.push_back(c.src);
+      it.nd.push_back(c.dst);
+    }
+  }
+}
+
+struct ServerReq : PipeItem {
   uint64_t id = 0;
   std::vector<dsm_change> ch;
-  std::vector<uint32_t> ps, pd, ns, nd;  // positive and negative records
   uint64_t bytes = 0;
   double t_submit = 0, t_start = 0, t_done = 0;
-  uint32_t queue_depth = 0, fused = 0, slot = 0;
+  uint32_t queue_depth = 0, fused = 0;
   int status = 1;  // 1 while pending or in flight
-  std::vector<uint64_t> pos, neg;
 };
 
 }  // namespace
@@ -1459,9 +1483,10 @@ void publish(dsm_server *sv, ServerReq *r, int status) {
 
 // One DFS round over the current graph: the positives of P (if any) and the
 // negatives of R (if any); then P's count slot is copied to the host. Waits
-// for the round; while it runs, computes R's delta into *d.
-int serve_round(dsm_server *sv, ServerReq *P, ServerReq *R, HostDelta *d) {
-  dsm_engine *e = sv->e;
+// for the round; while it runs, computes R's delta into *d. With `stats`, the
+// round's work counters and time go to the engine's statistics.
+int pipe_round(dsm_engine *e, PinnedBuf<uint32_t> &h_rec, DevBuf<uint32_t> &d_rec, PipeItem *P, PipeItem *R,
+               HostDelta *d, bool stats) {
   const uint32_t nq = (uint32_t)e->queries.size();
   int rc;
   const uint32_t np = P ? (uint32_t)P->ps.size() : 0u, nn = R ? (uint32_t)R->ns.size() : 0u;
@@ -1469,9 +1494,9 @@ int serve_round(dsm_server *sv, ServerReq *P, ServerReq *R, HostDelta *d) {
                                  e->stream));
   if (np + nn) {
     if ((rc = ensure_seed_cap(e, np + nn))) return rc;
-    HIPCHK(sv->h_rec.ensure(2 * (size_t)(np + nn) + 2));
-    HIPCHK(sv->d_rec.ensure(2 * (size_t)(np + nn) + 2));
-    uint32_t *h = sv->h_rec.p;
+    HIPCHK(h_rec.ensure(2 * (size_t)(np + nn) + 2));
+    HIPCHK(d_rec.ensure(2 * (size_t)(np + nn) + 2));
+    uint32_t *h = h_rec.p;
     if (np) {
       std::copy(P->ps.begin(), P->ps.end(), h);
       std::copy(P->pd.begin(), P->pd.end(), h + np);
@@ -1480,23 +1505,33 @@ int serve_round(dsm_server *sv, ServerReq *P, ServerReq *R, HostDelta *d) {
       std::copy(R->ns.begin(), R->ns.end(), h + 2 * np);
       std::copy(R->nd.begin(), R->nd.end(), h + 2 * np + nn);
     }
-    HIPCHK(hipMemcpyAsync(sv->d_rec.p, h, 2 * (size_t)(np + nn) * 4, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(d_rec.p, h, 2 * (size_t)(np + nn) * 4, hipMemcpyHostToDevice, e->stream));
     RoundPart parts[2];
     int nparts = 0;
-    if (np) parts[nparts++] = RoundPart{sv->d_rec.p, sv->d_rec.p + np, np, (P->slot << 1) | 1u};
-    if (nn) parts[nparts++] = RoundPart{sv->d_rec.p + 2 * np, sv->d_rec.p + 2 * np + nn, nn, R->slot << 1};
-    if ((rc = dfs_round(e, parts, nparts, nullptr, nullptr, nullptr))) return rc;
+    if (np) parts[nparts++] = RoundPart{d_rec.p, d_rec.p + np, np, (P->slot << 1) | 1u};
+    if (nn) parts[nparts++] = RoundPart{d_rec.p + 2 * np, d_rec.p + 2 * np + nn, nn, R->slot << 1};
+    if ((rc = dfs_round(e, parts, nparts, stats ? e->ev[9] : nullptr, stats ? e->ev[10] : nullptr,
+                        stats ? e->ev[11] : nullptr)))
+      return rc;
   }
   if (P)
     HIPCHK(hipMemcpyAsync(e->h_counts.p + (size_t)P->slot * nq * 2, e->counts.p + (size_t)P->slot * nq * 2,
                           (size_t)nq * 2 * 8, hipMemcpyDeviceToHost, e->stream));
-  if (R && d) host_delta(e, R->ch.data(), (uint32_t)R->ch.size(), *d);
+  if (R && d) host_delta(e, R->rec, R->nrec, *d);
   HIPCHK(hipStreamSynchronize(e->stream));
   if (np + nn) {
     const DfsCtl &c = *e->h_ctl.p;
-    if (c.seed_overflow.v) return e->fail_msg(DSM_ELIMIT, "seed table overflow in a served round");
+    if (c.seed_overflow.v) return e->fail_msg(DSM_ELIMIT, "seed table overflow in a pipelined round");
     if (c.abort.v)
       return e->fail_msg(DSM_EDEVICE, "DFS watchdog: a wave waited more than %u ms for work", e->watchdog_ms);
+    if (stats) {
+      uint64_t seeds = 0;
+      if ((rc = check_round(e, &seeds))) return rc;
+      e->stats.seeds_pos += seeds;
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e->ev[9], e->ev[11]));
+      e->stats.ms_dfs_pos += ms;
+    }
   }
   if (P) {
     P->pos.assign(nq, 0);
@@ -1508,6 +1543,10 @@ int serve_round(dsm_server *sv, ServerReq *P, ServerReq *R, HostDelta *d) {
     }
   }
   return DSM_OK;
+}
+
+int serve_round(dsm_server *sv, ServerReq *P, ServerReq *R, HostDelta *d) {
+  return pipe_round(sv->e, sv->h_rec, sv->d_rec, P, R, d, false);
 }
 
 void server_loop(dsm_server *sv) {
@@ -1545,15 +1584,7 @@ void server_loop(dsm_server *sv) {
       R->t_start = now_s();
       R->slot = P ? P->slot ^ 1u : 0u;
       R->fused = P ? 1u : 0u;
-      for (const dsm_change &c : R->ch) {
-        if (c.add) {
-          R->ps.push_back(c.src);
-          R->pd.push_back(c.dst);
-        } else if (e->opt.count_negatives) {
-          R->ns.push_back(c.src);
-          R->nd.push_back(c.dst);
-        }
-      }
+      pipe_split(e, *R);
     }
     HostDelta d;
     int rc = serve_round(sv, P, R, R ? &d : nullptr);
@@ -1612,6 +1643,8 @@ int dsm_server_submit(dsm_server *sv, const dsm_change *changes, uint32_t num_ch
     if (changes[i].src >= n || changes[i].dst >= n) return DSM_EINVAL;
   ServerReq *r = new ServerReq();
   r->ch.assign(changes, changes + num_changes);
+  r->rec = r->ch.data();
+  r->nrec = num_changes;
   r->bytes = request_bytes(sv->e, num_changes);
   r->t_submit = now_s();
   {
@@ -1657,6 +1690,65 @@ int dsm_server_wait(dsm_server *sv, uint64_t id, dsm_request_info *info, uint64_
 }
 
 const char *dsm_server_error(const dsm_server *sv) { return sv ? sv->err.c_str() : ""; }
+
+int dsm_process_batches(dsm_engine *e, const dsm_change *changes, const uint32_t *batch_offsets,
+                        uint32_t num_batches, uint64_t *pos, uint64_t *neg, double *done_s) {
+  if (!e || !batch_offsets || (!changes && batch_offsets[num_batches]) || !pos || !neg) return DSM_EINVAL;
+  if (e->tuple_cap) return e->fail_msg(DSM_EINVAL, "pipelined batches keep counts only (tuple_capacity must be 0)");
+  if (e->batch_pending) return e->fail_msg(DSM_EINVAL, "a batch is applied but not matched yet");
+  for (uint32_t b = 0; b < num_batches; b++)
+    if (batch_offsets[b + 1] < batch_offsets[b]) return e->fail_msg(DSM_EINVAL, "batch offsets decrease");
+  for (uint32_t i = 0; i < batch_offsets[num_batches]; i++)
+    if (changes[i].src >= e->n || changes[i].dst >= e->n)
+      return e->fail_msg(DSM_EINVAL, "change %u: endpoint out of range (%u,%u) for %u vertices", i,
+                         changes[i].src, changes[i].dst, e->n);
+  HIPCHK(hipSetDevice(e->opt.device));
+  e->stats = dsm_stats{};
+  const uint32_t nq = (uint32_t)e->queries.size();
+  std::vector<PipeItem> items(num_batches);
+  for (uint32_t b = 0; b < num_batches; b++) {
+    items[b].rec = changes + batch_offsets[b];
+    items[b].nrec = batch_offsets[b + 1] - batch_offsets[b];
+    pipe_split(e, items[b]);
+  }
+  auto publish_item = [&](uint32_t b) {
+    for (uint32_t q = 0; q < nq; q++) {
+      pos[(size_t)b * nq + q] = items[b].pos[q];
+      neg[(size_t)b * nq + q] = items[b].neg[q];
+    }
+    if (done_s) done_s[b] = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  };
+  int rc;
+  bool apply_timed = false;
+  auto take_apply_time = [&]() -> int {
+    if (!apply_timed) return DSM_OK;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
+    e->stats.ms_apply += ms;
+    apply_timed = false;
+    return DSM_OK;
+  };
+  PipeItem *P = nullptr;
+  for (uint32_t b = 0; b < num_batches; b++) {
+    PipeItem *R = &items[b];
+    R->slot = P ? P->slot ^ 1u : 0u;
+    HostDelta d;
+    if ((rc = pipe_round(e, e->pipe_h_rec, e->pipe_d_rec, P, R, &d, true))) return rc;
+    if ((rc = take_apply_time())) return rc;
+    if (P) publish_item(b - 1);
+    HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    if ((rc = apply_delta_and_refresh(e, d))) return rc;
+    HIPCHK(hipEventRecord(e->ev[2], e->stream));
+    apply_timed = true;
+    P = R;
+  }
+  if (P) {
+    if ((rc = pipe_round(e, e->pipe_h_rec, e->pipe_d_rec, P, nullptr, nullptr, true))) return rc;
+    if ((rc = take_apply_time())) return rc;
+    publish_item(num_batches - 1);
+  }
+  return DSM_OK;
+}
 
 void dsm_server_destroy(dsm_server *sv) {
   if (!sv) return;

@@ -37,7 +37,7 @@ EXPORTS = (
     "dsm_get_candidates", "dsm_num_vertices", "dsm_num_queries", "dsm_file_read", "dsm_file_free",
     "dsm_file_graph", "dsm_file_query", "dsm_file_num_batches", "dsm_file_batch", "dsm_file_error",
     "dsm_server_create", "dsm_server_submit", "dsm_server_wait", "dsm_server_error", "dsm_server_destroy",
-    "dsm_admit", "dsm_estimate_request_bytes", "dsm_autotune",
+    "dsm_admit", "dsm_estimate_request_bytes", "dsm_autotune", "dsm_process_batches",
 )
 
 
@@ -173,6 +173,8 @@ def lib():
     L.dsm_server_destroy.argtypes = [vp]
     L.dsm_admit.restype = C.c_int
     L.dsm_admit.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+    L.dsm_process_batches.restype = C.c_int
+    L.dsm_process_batches.argtypes = [vp, C.POINTER(Change), vp, C.c_uint32, vp, vp, vp]
     L.dsm_autotune.restype = C.c_int
     L.dsm_autotune.argtypes = [vp, C.POINTER(Change), C.c_uint32, C.POINTER(TuneReport)]
     L.dsm_estimate_request_bytes.restype = C.c_uint64
@@ -318,6 +320,22 @@ class Engine:
         self._check(lib().dsm_process_batch(self._h, rec.ctypes.data_as(C.POINTER(Change)), rec.shape[0],
                                             pos.ctypes.data, neg.ctypes.data))
         return pos, neg
+
+    def process_batches(self, batches):
+        """dsm_process_batches over a list of (add, src, dst, label) batches:
+        (pos u64[nb, nq], neg u64[nb, nq], done_s f64[nb])."""
+        recs = [changes_array(*b) for b in batches]
+        nb = len(recs)
+        off = np.zeros(nb + 1, np.uint32)
+        off[1:] = np.cumsum([r.shape[0] for r in recs])
+        allrec = np.concatenate(recs) if nb else np.zeros((0, 4), np.uint32)
+        allrec = np.ascontiguousarray(allrec if len(allrec) else np.zeros((1, 4), np.uint32))
+        pos = np.zeros((max(nb, 1), self.nq), np.uint64)
+        neg = np.zeros((max(nb, 1), self.nq), np.uint64)
+        done = np.zeros(max(nb, 1), np.float64)
+        self._check(lib().dsm_process_batches(self._h, allrec.ctypes.data_as(C.POINTER(Change)), off.ctypes.data,
+                                              nb, pos.ctypes.data, neg.ctypes.data, done.ctypes.data))
+        return pos[:nb], neg[:nb], done[:nb]
 
     def autotune(self, add, src, dst, label):
         """dsm_autotune over these records; returns the report as a dict."""

@@ -174,6 +174,19 @@ int dsm_match_batch(dsm_engine *e, uint64_t *pos_counts, uint64_t *neg_counts);
 int dsm_process_batch(dsm_engine *e, const dsm_change *changes, uint32_t num_changes,
                       uint64_t *pos_counts, uint64_t *neg_counts);
 
+/* A sequence of batches, each with the semantics of dsm_process_batch and in
+ * order, pipelined: each DFS launch matches the positives of batch b together
+ * with the negatives of batch b+1, which are evaluated on the same graph
+ * state, and batch b+1's host-side delta is computed while that launch runs.
+ * Batch b is changes[batch_offsets[b] .. batch_offsets[b+1]); the counts go
+ * to pos_counts / neg_counts [num_batches][num_queries]; done_s (optional)
+ * receives the steady-clock second at which each batch's counts were ready.
+ * Counts only (tuple_capacity must be 0). The statistics cover the whole
+ * call. No counterpart in the reference, which matches one batch per loop
+ * iteration (main.cu:152-348). */
+int dsm_process_batches(dsm_engine *e, const dsm_change *changes, const uint32_t *batch_offsets,
+                        uint32_t num_batches, uint64_t *pos_counts, uint64_t *neg_counts, double *done_s);
+
 /* Tuples found by the last dsm_match_batch (needs tuple_capacity > 0). Each
  * tuple is (query, sign, f(u_0) .. f(u_{n-1})) in DSM_MAX_QUERY_VERTICES + 2
  * uint32 words (sign 1 = positive, 0 = negative; unused words are

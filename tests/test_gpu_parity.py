@@ -459,3 +459,44 @@ def test_label_sorted_copy_of_huge_hub():
     nb = rng.choice(np.setdiff1d(np.arange(1, n), hub_nbrs), k, replace=False).astype(np.uint32)
     batches.append((np.ones(k, np.uint8), np.zeros(k, np.uint32), nb, rng.integers(0, nl, k).astype(np.uint32)))
     compare_stream(n, labels, eu, ev, queries, batches, relabel=True, check_graph=True)
+
+
+@pytest.mark.parametrize("relabel", [True, False])
+def test_process_batches_pipelined_equals_one_by_one(relabel):
+    """dsm_process_batches (the positives of batch b share a launch with the
+    negatives of batch b+1) against dsm_process_batch one batch at a time and
+    the oracle, with empty, insert-only, delete-only and mixed batches."""
+    say("case: pipelined batches, relabel=%s" % relabel)
+    rng = np.random.default_rng(1616 if relabel else 1717)
+    n, m, nl = 400, 4000, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple("triangle", [0, 1, 0]), qtuple("house", [0, 1, 0, 1, 0]), qtuple("path6", [0, 1, 0, 1, 0, 1])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batches = [conflict_free_batch(n, 60, present, rng),
+               conflict_free_batch(n, 0, present, rng),
+               conflict_free_batch(n, 30, present, rng, p_add=1.0),
+               conflict_free_batch(n, 30, present, rng, p_add=0.0),
+               conflict_free_batch(n, 90, present, rng),
+               conflict_free_batch(n, 1, present, rng)]
+    g = O.Graph(n, labels, eu, ev)
+    want = O.run_stream(g, oracle_queries(queries), batches, relabel=relabel)
+    one = E.Engine((n, labels, eu, ev), queries, relabel=relabel)
+    pipe = E.Engine((n, labels, eu, ev), queries, relabel=relabel)
+    try:
+        pos, neg, done = pipe.process_batches(batches)
+        assert np.all(np.diff(done) >= 0)
+        for b, bt in enumerate(batches):
+            p1, n1 = one.process_batch(*bt)
+            assert pos[b].tolist() == p1.tolist() == want[b]["pos"], "batch %d positives" % b
+            assert neg[b].tolist() == n1.tolist() == want[b]["neg"], "batch %d negatives" % b
+        off, nb, _ = pipe.graph()
+        ooff, onb, _ = g.export()
+        assert off.tolist() == ooff.tolist() and np.array_equal(nb, onb)
+        # and the engine continues with single batches afterwards
+        more = conflict_free_batch(n, 40, present, rng)
+        w = O.run_stream(g, oracle_queries(queries), [more], relabel=relabel)[0]
+        p2, n2 = pipe.process_batch(*more)
+        assert p2.tolist() == w["pos"] and n2.tolist() == w["neg"]
+    finally:
+        one.close()
+        pipe.close()
