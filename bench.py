@@ -88,11 +88,12 @@ def _agg_add(agg, st):
 
 
 def run_batched(w, warmup, steps, engine_kw, autotune=True, pipelined=True):
-    """One engine for all queries. Warm-up batches go through dsm_process_batch;
-    the timed ones through dsm_process_batches (pipelined: the positives of a
-    batch share a launch with the next batch's negatives) or, with
-    pipelined=False, dsm_process_batch one at a time. A pipelined batch's
-    latency is the time between its counts and the previous batch's."""
+    """One engine for all queries. Warm-up and timed batches go through
+    dsm_process_batches (pipelined: the positives of a batch share a launch
+    with the next batch's negatives, and the next batch is applied to the
+    second copy of the graph meanwhile) or, with pipelined=False,
+    dsm_process_batch one at a time. A pipelined batch's latency is the time
+    between its counts and the previous batch's."""
     from dsm_amd import engine as E
     t0 = time.perf_counter()
     eng = E.Engine((w.n, w.labels, w.eu, w.ev), qtuples(w), **engine_kw)
@@ -108,7 +109,12 @@ def run_batched(w, warmup, steps, engine_kw, autotune=True, pipelined=True):
     pos_tot = neg_tot = 0
     per_batch = []
     try:
-        for b in range(warmup):
+        if pipelined and warmup:
+            t = time.perf_counter()
+            pos, neg, _ = eng.process_batches([w.batch(b) for b in range(warmup)])
+            per_batch += [(pos[b].copy(), neg[b].copy()) for b in range(warmup)]
+            log("warmup: %d pipelined batches %.2f ms" % (warmup, (time.perf_counter() - t) * 1e3))
+        for b in range(0 if pipelined else warmup):
             t = time.perf_counter()
             pos, neg = eng.process_batch(*w.batch(b))
             per_batch.append((pos.copy(), neg.copy()))
@@ -207,6 +213,11 @@ def run_served(w, rates, first_batch, engine_kw, seed=7, max_requests=160, offer
 
     out = []
     try:
+        # one untimed request: the server's first update allocates the
+        # second copy of the graph
+        st = srv.wait(srv.submit(*take(64)))[0]
+        if st != E.DSM_OK:
+            raise RuntimeError("served warm-up request failed: %s" % srv.error())
         for rate in rates:
             nreq = int(min(max_requests, max(20, rate * offered_s / mean_size)))
             sizes = np.exp(rng.uniform(0, np.log(hi + 1), nreq)).astype(np.int64).clip(1, hi)
@@ -365,6 +376,8 @@ def main():
     ap.add_argument("--donate-min-n2", type=int, default=0)
     ap.add_argument("--leaf-segments", type=int, default=1)
     ap.add_argument("--poll-max", type=int, default=0)
+    ap.add_argument("--overlap-apply", type=int, default=1,
+                    help="1: pipelined batches update a second copy of the graph during the round")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run one D2D copy of this many bytes")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -383,7 +396,7 @@ def main():
                  "blocks_per_cu": args.blocks_per_cu, "lane_work": args.lane_work,
                  "leaf_sets": bool(args.leaf_sets), "donate_every": args.donate_every,
                  "donate_min_n2": args.donate_min_n2, "leaf_segments": bool(args.leaf_segments),
-                 "poll_max": args.poll_max}
+                 "poll_max": args.poll_max, "overlap_apply": bool(args.overlap_apply)}
 
     batch = int(w.bstart[1] - w.bstart[0])
     one = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune, pipelined=False)

@@ -143,6 +143,19 @@ struct dsm_engine {
   DevBuf<uint32_t> pipe_d_rec;
   bool leaf_segments = true;
   uint32_t poll_max = 16;
+  // Second copy of the graph: a pipelined round matches on the engine's copy
+  // while the next batch is applied to this one on stream2; then they swap.
+  struct GraphCopy {
+    DevBuf<uint32_t> pool, start, deg, cap, lpool, loff;
+    DevBuf<uint8_t> lab;
+    DevBuf<uint64_t> bm;
+    DevBuf<ApplyCtl> actl;
+    unsigned long long pool_cap = 0;
+  } shadow;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_pre = nullptr;  // all work on stream before a pipelined round
+  bool overlap_apply = true;
+  bool lpool_stale = false;  // the pool grew during the current update
   // per-wave task time of the last DFS launch (balance statistics)
   DevBuf<unsigned long long> wave_busy;
   PinnedBuf<unsigned long long> h_wave_busy;
@@ -170,6 +183,8 @@ struct dsm_engine {
       if (tuples_host) (void)hipHostFree(tuples);
       else (void)hipFree(tuples);
     }
+    if (ev_pre) (void)hipEventDestroy(ev_pre);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -477,6 +492,7 @@ int upload_graph(dsm_engine *e, const dsm_graph_desc *g) {
   ApplyCtl a0{};
   a0.pool_used = total;
   HIPCHK(hipMemcpy(e->actl.p, &a0, sizeof a0, hipMemcpyHostToDevice));
+  *e->h_actl.p = a0;
   HIPCHK(e->dirty_bits.alloc((n + 31) / 32));
   HIPCHK(hipMemset(e->dirty_bits.p, 0, ((n + 31) / 32) * 4));
   HIPCHK(e->dirty_list.alloc(n));
@@ -866,13 +882,22 @@ int grow_pool(dsm_engine *e, unsigned long long extra) {
   std::swap(e->start.n, nst.n);
   e->pool_cap = ncap;
   if (int rc = size_huge_sort(e)) return rc;
+  // every label-sorted copy moves with the pool: the update that grew it
+  // rebuilds them all once its commits are done (the rebuild uses the merge
+  // scratch, which still holds the lists of the commits that failed)
   HIPCHK(e->lpool.alloc(ncap));
-  // scratch must hold one copy of every list
+  e->lpool_stale = true;
+  // scratch must hold one copy of every list; the failed commits' lists move
+  // with it
   if (e->scratch_cap < ncap + (1ull << 20)) {
+    DevBuf<uint32_t> ns;
+    HIPCHK(ns.alloc(ncap + (1ull << 20)));
+    HIPCHK(hipMemcpyAsync(ns.p, e->scratch.p, e->scratch_cap * 4, hipMemcpyDeviceToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::swap(e->scratch.p, ns.p);
+    std::swap(e->scratch.n, ns.n);
     e->scratch_cap = ncap + (1ull << 20);
-    HIPCHK(e->scratch.alloc(e->scratch_cap));
   }
-  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
   ApplyCtl a = *e->h_actl.p;
   a.pool_used = total;
   a.overflow = 0;
@@ -950,8 +975,14 @@ int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d) {
   HIPCHK(launch_ctl_reset(nullptr, e->actl.p, nullptr, nullptr, 0, e->stream));
   if ((rc = do_apply_delta(e, d))) return rc;
   // the dirty list (touched vertices, relabelled ones and their neighbours) is
-  // exactly the set whose label-sorted copies changed
-  HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, e->dirty_list.p, &e->actl.p->ndirty, e->huge_sort(), e->stream));
+  // exactly the set whose label-sorted copies changed, unless the pool grew
+  if (e->lpool_stale) {
+    HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
+    e->lpool_stale = false;
+  } else {
+    HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, e->dirty_list.p, &e->actl.p->ndirty, e->huge_sort(),
+                         e->stream));
+  }
   HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
   return DSM_OK;
 }
@@ -967,6 +998,8 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   int rc = 0;
   HIPCHK(hipSetDevice(e->opt.device));
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&e->ev_pre, hipEventDisableTiming));
   HIPCHK(hipDeviceGetAttribute(&e->num_cu, hipDeviceAttributeMultiprocessorCount, e->opt.device));
   HIPCHK(hipDeviceGetAttribute(&e->wall_khz, hipDeviceAttributeWallClockRate, e->opt.device));
   if (e->wall_khz <= 0) e->wall_khz = 100000;
@@ -1006,6 +1039,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   e->graph_rounds = e->opt.graph_rounds != 0;
   e->leaf_segments = e->opt.leaf_segments != 0;
   e->poll_max = e->opt.poll_max ? std::min<uint32_t>(e->opt.poll_max, 16u) : 16u;
+  e->overlap_apply = e->opt.overlap_apply != 0;
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
   HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -1027,6 +1061,7 @@ void dsm_options_default(dsm_options *opt) {
   opt->leaf_sets = 1;
   opt->graph_rounds = 1;
   opt->leaf_segments = 1;
+  opt->overlap_apply = 1;
 }
 
 const char *dsm_last_error(const dsm_engine *e) {
@@ -1485,11 +1520,95 @@ void publish(dsm_server *sv, ServerReq *r, int status) {
 // negatives of R (if any); then P's count slot is copied to the host. Waits
 // for the round; while it runs, computes R's delta into *d. With `stats`, the
 // round's work counters and time go to the engine's statistics.
+template <typename T>
+void swap_buf(DevBuf<T> &a, DevBuf<T> &b) {
+  std::swap(a.p, b.p);
+  std::swap(a.n, b.n);
+}
+
+// Exchanges the engine's graph with the second copy.
+void swap_graph(dsm_engine *e) {
+  auto &c = e->shadow;
+  swap_buf(e->pool, c.pool);
+  swap_buf(e->start, c.start);
+  swap_buf(e->deg, c.deg);
+  swap_buf(e->cap, c.cap);
+  swap_buf(e->lpool, c.lpool);
+  swap_buf(e->loff, c.loff);
+  swap_buf(e->lab, c.lab);
+  swap_buf(e->bm, c.bm);
+  swap_buf(e->actl, c.actl);
+  std::swap(e->pool_cap, c.pool_cap);
+}
+
+template <typename T>
+hipError_t copy_buf(DevBuf<T> &dst, const DevBuf<T> &src, size_t count, hipStream_t s) {
+  if (dst.n < src.n) {
+    hipError_t err = dst.alloc(src.n);
+    if (err != hipSuccess) return err;
+  }
+  return count ? hipMemcpyAsync(dst.p, src.p, count * sizeof(T), hipMemcpyDeviceToDevice, s) : hipSuccess;
+}
+
+// Makes the engine's graph a copy of `c` on the engine's stream. The pool
+// and the label-sorted copies are copied up to the pool's used end (the
+// host's ApplyCtl mirror holds it after every update).
+int copy_graph(dsm_engine *e, const dsm_engine::GraphCopy &c) {
+  unsigned long long used = e->h_actl.p->pool_used;
+  if (used == 0 || used > c.pool_cap) used = c.pool_cap;
+  HIPCHK(copy_buf(e->pool, c.pool, used, e->stream));
+  HIPCHK(copy_buf(e->lpool, c.lpool, used, e->stream));
+  HIPCHK(copy_buf(e->start, c.start, c.start.n, e->stream));
+  HIPCHK(copy_buf(e->deg, c.deg, c.deg.n, e->stream));
+  HIPCHK(copy_buf(e->cap, c.cap, c.cap.n, e->stream));
+  HIPCHK(copy_buf(e->loff, c.loff, c.loff.n, e->stream));
+  HIPCHK(copy_buf(e->lab, c.lab, c.lab.n, e->stream));
+  HIPCHK(copy_buf(e->bm, c.bm, c.bm.n, e->stream));
+  HIPCHK(copy_buf(e->actl, c.actl, 1, e->stream));
+  e->pool_cap = c.pool_cap;
+  return DSM_OK;
+}
+
+// Applies one batch to the second copy of the graph while a round that was
+// enqueued on `stream` matches on the first: the copies swap, the new current
+// one becomes a copy of the old one (after ev_pre, i.e. after every update
+// enqueued before the round) and takes the batch, all on stream2. Returns
+// once the update is done; the copies stay swapped.
+int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats) {
+  std::swap(e->stream, e->stream2);
+  swap_graph(e);
+  int rc = DSM_OK;
+  auto run = [&]() -> int {
+    HIPCHK(hipStreamWaitEvent(e->stream, e->ev_pre, 0));
+    if (stats) HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    if (int r = copy_graph(e, e->shadow)) return r;
+    if (int r = apply_delta_and_refresh(e, d)) return r;
+    if (stats) HIPCHK(hipEventRecord(e->ev[2], e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (stats) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
+      e->stats.ms_apply += ms;
+    }
+    return DSM_OK;
+  };
+  rc = run();
+  std::swap(e->stream, e->stream2);
+  return rc;
+}
+
+// One pipelined round: the positives of P and the negatives of R in one DFS
+// launch on the current graph. With `applied` given and overlap_apply on, R's
+// update is applied to the second copy of the graph meanwhile and *applied is
+// set; otherwise the caller applies it after the round.
 int pipe_round(dsm_engine *e, PinnedBuf<uint32_t> &h_rec, DevBuf<uint32_t> &d_rec, PipeItem *P, PipeItem *R,
-               HostDelta *d, bool stats) {
+               HostDelta *d, bool stats, bool *applied) {
   const uint32_t nq = (uint32_t)e->queries.size();
   int rc;
   const uint32_t np = P ? (uint32_t)P->ps.size() : 0u, nn = R ? (uint32_t)R->ns.size() : 0u;
+  const bool overlap = applied && R && d && e->overlap_apply;
+  if (applied) *applied = false;
+  if (overlap) HIPCHK(hipEventRecord(e->ev_pre, e->stream));
   if (R) HIPCHK(launch_ctl_reset(nullptr, nullptr, nullptr, e->counts.p + (size_t)R->slot * nq * 2, nq * 2,
                                  e->stream));
   if (np + nn) {
@@ -1518,6 +1637,10 @@ int pipe_round(dsm_engine *e, PinnedBuf<uint32_t> &h_rec, DevBuf<uint32_t> &d_re
     HIPCHK(hipMemcpyAsync(e->h_counts.p + (size_t)P->slot * nq * 2, e->counts.p + (size_t)P->slot * nq * 2,
                           (size_t)nq * 2 * 8, hipMemcpyDeviceToHost, e->stream));
   if (R && d) host_delta(e, R->rec, R->nrec, *d);
+  if (overlap) {
+    if ((rc = overlap_apply(e, *d, stats))) return rc;
+    *applied = true;
+  }
   HIPCHK(hipStreamSynchronize(e->stream));
   if (np + nn) {
     const DfsCtl &c = *e->h_ctl.p;
@@ -1545,8 +1668,8 @@ int pipe_round(dsm_engine *e, PinnedBuf<uint32_t> &h_rec, DevBuf<uint32_t> &d_re
   return DSM_OK;
 }
 
-int serve_round(dsm_server *sv, ServerReq *P, ServerReq *R, HostDelta *d) {
-  return pipe_round(sv->e, sv->h_rec, sv->d_rec, P, R, d, false);
+int serve_round(dsm_server *sv, ServerReq *P, ServerReq *R, HostDelta *d, bool *applied) {
+  return pipe_round(sv->e, sv->h_rec, sv->d_rec, P, R, d, false, applied);
 }
 
 void server_loop(dsm_server *sv) {
@@ -1587,9 +1710,10 @@ void server_loop(dsm_server *sv) {
       pipe_split(e, *R);
     }
     HostDelta d;
-    int rc = serve_round(sv, P, R, R ? &d : nullptr);
+    bool applied = false;
+    int rc = serve_round(sv, P, R, R ? &d : nullptr, &applied);
     if (P) publish(sv, P, rc);
-    if (rc == DSM_OK && R) rc = apply_delta_and_refresh(e, d);
+    if (rc == DSM_OK && R && !applied) rc = apply_delta_and_refresh(e, d);
     if (rc != DSM_OK) {
       std::lock_guard<std::mutex> lk(sv->mu);
       sv->failed = rc;
@@ -1733,20 +1857,25 @@ int dsm_process_batches(dsm_engine *e, const dsm_change *changes, const uint32_t
     PipeItem *R = &items[b];
     R->slot = P ? P->slot ^ 1u : 0u;
     HostDelta d;
-    if ((rc = pipe_round(e, e->pipe_h_rec, e->pipe_d_rec, P, R, &d, true))) return rc;
+    bool applied = false;
+    if ((rc = pipe_round(e, e->pipe_h_rec, e->pipe_d_rec, P, R, &d, true, &applied))) return rc;
     if ((rc = take_apply_time())) return rc;
     if (P) publish_item(b - 1);
-    HIPCHK(hipEventRecord(e->ev[0], e->stream));
-    if ((rc = apply_delta_and_refresh(e, d))) return rc;
-    HIPCHK(hipEventRecord(e->ev[2], e->stream));
-    apply_timed = true;
+    if (!applied) {
+      HIPCHK(hipEventRecord(e->ev[0], e->stream));
+      if ((rc = apply_delta_and_refresh(e, d))) return rc;
+      HIPCHK(hipEventRecord(e->ev[2], e->stream));
+      apply_timed = true;
+    }
     P = R;
   }
   if (P) {
-    if ((rc = pipe_round(e, e->pipe_h_rec, e->pipe_d_rec, P, nullptr, nullptr, true))) return rc;
+    if ((rc = pipe_round(e, e->pipe_h_rec, e->pipe_d_rec, P, nullptr, nullptr, true, nullptr))) return rc;
     if ((rc = take_apply_time())) return rc;
     publish_item(num_batches - 1);
   }
+  e->stats.pool_entries_used = e->h_actl.p->pool_used;
+  e->stats.pool_entries_capacity = e->pool_cap;
   return DSM_OK;
 }
 

@@ -94,6 +94,11 @@ typedef struct {
   int graph_rounds;          /* 1: each DFS round (control reset, seeding, DFS, status
                                 copy) is one HIP graph launch, built once and updated
                                 per round; 0: separate launches */
+  int overlap_apply;         /* 1: pipelined batches (dsm_process_batches, the served
+                                stream) apply the next batch to a second copy of the
+                                graph while the round matches on the first, which
+                                doubles the graph's device memory; 0: apply after the
+                                round */
 } dsm_options;
 
 /* Per-batch statistics of the last dsm_apply_batch + dsm_match_batch. */

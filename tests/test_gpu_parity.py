@@ -193,8 +193,9 @@ def test_dense_graph_work_donation():
 
 
 def test_hub_vertex_many_changes_and_pool_growth():
-    """One vertex gets > 1024 inserts in one batch (several apply rounds) and
-    the pool must grow (relocation beyond the initial reserve)."""
+    """One vertex gets > 1024 inserts in one batch (several apply rounds),
+    then every vertex gains edges beyond its slack (relocation of every list
+    into the pool's reserve)."""
     say("case: hub vertex with 3000 inserts in one batch")
     rng = np.random.default_rng(404)
     n = 5000
@@ -461,12 +462,13 @@ def test_label_sorted_copy_of_huge_hub():
     compare_stream(n, labels, eu, ev, queries, batches, relabel=True, check_graph=True)
 
 
-@pytest.mark.parametrize("relabel", [True, False])
-def test_process_batches_pipelined_equals_one_by_one(relabel):
+@pytest.mark.parametrize("relabel,overlap", [(True, True), (False, True), (True, False)])
+def test_process_batches_pipelined_equals_one_by_one(relabel, overlap):
     """dsm_process_batches (the positives of batch b share a launch with the
-    negatives of batch b+1) against dsm_process_batch one batch at a time and
-    the oracle, with empty, insert-only, delete-only and mixed batches."""
-    say("case: pipelined batches, relabel=%s" % relabel)
+    negatives of batch b+1; with overlap, batch b+1 is applied to the second
+    copy of the graph meanwhile) against dsm_process_batch one batch at a time
+    and the oracle, with empty, insert-only, delete-only and mixed batches."""
+    say("case: pipelined batches, relabel=%s overlap=%s" % (relabel, overlap))
     rng = np.random.default_rng(1616 if relabel else 1717)
     n, m, nl = 400, 4000, 2
     labels, eu, ev = random_graph(n, m, nl, rng)
@@ -481,7 +483,7 @@ def test_process_batches_pipelined_equals_one_by_one(relabel):
     g = O.Graph(n, labels, eu, ev)
     want = O.run_stream(g, oracle_queries(queries), batches, relabel=relabel)
     one = E.Engine((n, labels, eu, ev), queries, relabel=relabel)
-    pipe = E.Engine((n, labels, eu, ev), queries, relabel=relabel)
+    pipe = E.Engine((n, labels, eu, ev), queries, relabel=relabel, overlap_apply=overlap)
     try:
         pos, neg, done = pipe.process_batches(batches)
         assert np.all(np.diff(done) >= 0)
@@ -500,3 +502,61 @@ def test_process_batches_pipelined_equals_one_by_one(relabel):
     finally:
         one.close()
         pipe.close()
+
+
+@pytest.mark.parametrize("overlap", [True, False])
+def test_process_batches_grows_pool(overlap):
+    """Pipelined batches, where one update relocates nearly every
+    list past the pool's slack. The pool starts at 1.5x the initial lists
+    (about 6 entries per vertex here) plus 2^20 entries. 600,000 inserts raise
+    the mean degree from 2 to 14, and the relocated lists need about 25
+    entries per vertex, so the pool grows in the middle of the update (with
+    overlap: the copy being updated on the second stream, while the other
+    copy is matched). Counts, the graph after
+    the stream, and a later pipelined call (the other copy, grown in turn)
+    are checked against the oracle."""
+    say("case: pipelined batches, pool growth, overlap=%s" % overlap)
+    rng = np.random.default_rng(1818)
+    n, m, nl = 100000, 100000, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple("triangle", [0, 1, 0]), qtuple("square", [0, 1, 0, 1])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    k = 600000
+    a = rng.integers(0, n, 2 * k)
+    b = rng.integers(0, n, 2 * k)
+    keys = np.unique(np.minimum(a, b)[a != b].astype(np.int64) * n + np.maximum(a, b)[a != b])
+    old = np.array(sorted(x * n + y for x, y in present), np.int64)
+    keys = keys[~np.isin(keys, old)]
+    keys = rng.permutation(keys)[:k]
+    assert len(keys) == k
+    for key in keys.tolist():
+        present.add((key // n, key % n))
+    big = (np.ones(k, np.uint8), (keys // n).astype(np.uint32), (keys % n).astype(np.uint32),
+           rng.integers(0, nl, k).astype(np.uint32))
+    first = [conflict_free_batch(n, 2000, present, rng), big, conflict_free_batch(n, 3000, present, rng)]
+    then = [conflict_free_batch(n, 3000, present, rng), conflict_free_batch(n, 1000, present, rng)]
+    g = O.Graph(n, labels, eu, ev)
+    want = O.run_stream(g, oracle_queries(queries), first, relabel=True)
+    say("  oracle done")
+    eng = E.Engine((n, labels, eu, ev), queries, relabel=True, overlap_apply=overlap)
+    try:
+        pos, neg, _ = eng.process_batches(first[:1])
+        cap0 = eng.stats()["pool_entries_capacity"]
+        p2, n2, _ = eng.process_batches(first[1:])
+        pos, neg = np.concatenate([pos, p2]), np.concatenate([neg, n2])
+        assert eng.stats()["pool_entries_capacity"] > cap0, "the pool did not grow"
+        for bi in range(len(first)):
+            assert pos[bi].tolist() == want[bi]["pos"], "batch %d positives" % bi
+            assert neg[bi].tolist() == want[bi]["neg"], "batch %d negatives" % bi
+        off, nb, _ = eng.graph()
+        ooff, onb, _ = g.export()
+        assert off.tolist() == ooff.tolist() and np.array_equal(nb, onb)
+        want2 = O.run_stream(g, oracle_queries(queries), then, relabel=True)
+        pos, neg, _ = eng.process_batches(then)
+        for bi in range(len(then)):
+            assert pos[bi].tolist() == want2[bi]["pos"] and neg[bi].tolist() == want2[bi]["neg"], "later %d" % bi
+        off, nb, _ = eng.graph()
+        ooff, onb, _ = g.export()
+        assert off.tolist() == ooff.tolist() and np.array_equal(nb, onb)
+    finally:
+        eng.close()
