@@ -362,6 +362,8 @@ def main():
     ap.add_argument("--workload", default="rmat1m_q64")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-one-by-one", action="store_true",
+                    help="skip the leg that calls dsm_process_batch once per batch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -399,9 +401,12 @@ def main():
                  "poll_max": args.poll_max, "overlap_apply": bool(args.overlap_apply)}
 
     batch = int(w.bstart[1] - w.bstart[0])
-    one = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune, pipelined=False)
+    # batches one at a time have no update to overlap: no second copy of the graph
+    one = None if args.no_one_by_one else run_batched(w, args.warmup, args.steps,
+                                                      dict(engine_kw, overlap_apply=False),
+                                                      autotune=not args.no_autotune, pipelined=False)
     bat = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune, pipelined=True)
-    for b, ((p1, n1), (p2, n2)) in enumerate(zip(one["per_batch"], bat["per_batch"])):
+    for b, ((p1, n1), (p2, n2)) in enumerate(zip(one["per_batch"] if one else [], bat["per_batch"])):
         assert np.array_equal(p1, p2) and np.array_equal(n1, n2), "pipelined and one-by-one differ at batch %d" % b
     total_s = sum(bat["lat"])
     value = bat["records"] / total_s
@@ -441,13 +446,14 @@ def main():
                                   "launch per graph state)" % (w.name, GRAPH_TEXT.get(w.name, ""), batch,
                                                                 len(w.queries)),
                       "batch": batch, "queries": len(w.queries),
-                      "mode": "batched, pipelined (a batch's positives share a launch with the next batch's negatives)",
+                      "mode": "batched, pipelined (a batch's positives share a launch with the next batch's negatives; the next batch is applied to a second copy of the graph meanwhile)",
                       "parallelism": "single GPU", "tuned": bat["tuned"]},
            "p50_batch_ms": round(pct(bat["lat"], 50) * 1e3, 3), "p99_batch_ms": round(pct(bat["lat"], 99) * 1e3, 3),
            "one_batch_per_call": {"value": round(one["records"] / sum(one["lat"]), 1),
                                   "p50_batch_ms": round(pct(one["lat"], 50) * 1e3, 3),
                                   "p99_batch_ms": round(pct(one["lat"], 99) * 1e3, 3),
-                                  "ms_dfs_per_batch": round(one["agg"]["ms_dfs"] / len(one["lat"]), 3)},
+                                  "ms_dfs_per_batch": round(one["agg"]["ms_dfs"] / len(one["lat"]), 3)}
+           if one else None,
            "matches_per_s": round((bat["pos"] + bat["neg"]) / total_s, 1),
            "stages_ms_per_batch": {k: round(agg[k] / len(bat["lat"]), 3) for k in ("ms_apply", "ms_filter",
                                                                                "ms_seed", "ms_dfs")},

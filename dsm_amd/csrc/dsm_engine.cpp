@@ -1046,6 +1046,89 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   return DSM_OK;
 }
 
+template <typename T>
+void swap_buf(DevBuf<T> &a, DevBuf<T> &b) {
+  std::swap(a.p, b.p);
+  std::swap(a.n, b.n);
+}
+
+// Exchanges the engine's graph with the second copy.
+void swap_graph(dsm_engine *e) {
+  auto &c = e->shadow;
+  swap_buf(e->pool, c.pool);
+  swap_buf(e->start, c.start);
+  swap_buf(e->deg, c.deg);
+  swap_buf(e->cap, c.cap);
+  swap_buf(e->lpool, c.lpool);
+  swap_buf(e->loff, c.loff);
+  swap_buf(e->lab, c.lab);
+  swap_buf(e->bm, c.bm);
+  swap_buf(e->actl, c.actl);
+  std::swap(e->pool_cap, c.pool_cap);
+}
+
+template <typename T>
+hipError_t copy_buf(DevBuf<T> &dst, const DevBuf<T> &src, size_t count, hipStream_t s) {
+  if (dst.n < src.n) {
+    hipError_t err = dst.alloc(src.n);
+    if (err != hipSuccess) return err;
+  }
+  return count ? hipMemcpyAsync(dst.p, src.p, count * sizeof(T), hipMemcpyDeviceToDevice, s) : hipSuccess;
+}
+
+// Makes the engine's graph a copy of `c` on the engine's stream. The pool
+// and the label-sorted copies are copied up to the pool's used end (the
+// host's ApplyCtl mirror holds it after every update).
+int copy_graph(dsm_engine *e, const dsm_engine::GraphCopy &c) {
+  unsigned long long used = e->h_actl.p->pool_used;
+  if (used == 0 || used > c.pool_cap) used = c.pool_cap;
+  HIPCHK(copy_buf(e->pool, c.pool, used, e->stream));
+  HIPCHK(copy_buf(e->lpool, c.lpool, used, e->stream));
+  HIPCHK(copy_buf(e->start, c.start, c.start.n, e->stream));
+  HIPCHK(copy_buf(e->deg, c.deg, c.deg.n, e->stream));
+  HIPCHK(copy_buf(e->cap, c.cap, c.cap.n, e->stream));
+  HIPCHK(copy_buf(e->loff, c.loff, c.loff.n, e->stream));
+  HIPCHK(copy_buf(e->lab, c.lab, c.lab.n, e->stream));
+  HIPCHK(copy_buf(e->bm, c.bm, c.bm.n, e->stream));
+  HIPCHK(copy_buf(e->actl, c.actl, 1, e->stream));
+  e->pool_cap = c.pool_cap;
+  return DSM_OK;
+}
+
+// Applies one batch to the second copy of the graph while a round that was
+// enqueued on `stream` matches on the first: the copies swap, the new current
+// one becomes a copy of the old one (after ev_pre, i.e. after every update
+// enqueued before the round) and takes the batch, all on stream2. Returns
+// once the update is done; the copies stay swapped, or with keep = false
+// swap back (the update is dropped: the tuner's load).
+int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats, bool keep = true) {
+  const ApplyCtl mirror = *e->h_actl.p;
+  std::swap(e->stream, e->stream2);
+  swap_graph(e);
+  int rc = DSM_OK;
+  auto run = [&]() -> int {
+    HIPCHK(hipStreamWaitEvent(e->stream, e->ev_pre, 0));
+    if (stats) HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    if (int r = copy_graph(e, e->shadow)) return r;
+    if (int r = apply_delta_and_refresh(e, d)) return r;
+    if (stats) HIPCHK(hipEventRecord(e->ev[2], e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (stats) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
+      e->stats.ms_apply += ms;
+    }
+    return DSM_OK;
+  };
+  rc = run();
+  std::swap(e->stream, e->stream2);
+  if (!keep) {
+    swap_graph(e);
+    *e->h_actl.p = mirror;
+  }
+  return rc;
+}
+
 }  // namespace
 
 // =============================================================== C ABI
@@ -1277,12 +1360,22 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
   if ((rc = ensure_seed_cap(e, k))) return rc;
   DevBuf<uint32_t> &rb = e->rec_neg;
   if ((rc = upload_records(e, s, d, rb))) return rc;
+  // the staging buffer is free again before an update (below) refills it
+  HIPCHK(hipStreamSynchronize(e->stream));
   struct Cfg {
     uint32_t wpb, bpc, dmin, lw;
     bool ls, seg;
     uint32_t dev;
   };
   auto max_bpc = [&](uint32_t wpb) { return (uint32_t)std::max(1, dfs_max_blocks_per_cu(wpb, e->max_qn)); };
+  // With overlap_apply, pipelined rounds run beside the next batch's update
+  // (on the second copy of the graph), which competes for the CUs: each trial
+  // round then runs beside the update of the first quarter of the sample (the
+  // sample being a few batches), applied to the second copy and dropped, and
+  // a trial's time is the makespan of both.
+  HostDelta load;
+  const bool loaded = e->overlap_apply && !e->tuple_cap && k >= 4;
+  if (loaded) host_delta(e, sample, k / 4, load);
   // times one sample round under c and checks its counts against the first
   std::vector<unsigned long long> ref_counts;
   uint32_t trials = 0;
@@ -1294,14 +1387,23 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     e->leaf_sets = c.ls;
     e->leaf_segments = c.seg;
     e->donate_every = c.dev;
+    if (loaded) HIPCHK(hipEventRecord(e->ev_pre, e->stream));
     HIPCHK(launch_ctl_reset(nullptr, nullptr, e->tuple_count.p, e->counts.p, nq * 2, e->stream));
     int rc2 = run_round(e, rb.p, rb.p + k, k, 0, e->ev[9], e->ev[10], e->ev[11]);
     if (rc2) return rc2;
+    if (loaded && (rc2 = overlap_apply(e, load, true, false))) return rc2;
     HIPCHK(hipMemcpyAsync(e->h_counts.p, e->counts.p, (size_t)nq * 2 * 8, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     uint64_t seeds = 0;
     if ((rc2 = check_round(e, &seeds))) return rc2;
     HIPCHK(hipEventElapsedTime(&ms, e->ev[10], e->ev[11]));
+    if (loaded) {
+      // the makespan of the round and the update beside it: a shape that
+      // leaves the update no room on the CUs delays it past the round
+      float ua = 0;
+      HIPCHK(hipEventElapsedTime(&ua, e->ev[10], e->ev[2]));
+      ms = std::max(ms, ua);
+    }
     std::vector<unsigned long long> cnt(e->h_counts.p, e->h_counts.p + 2 * (size_t)nq);
     if (ref_counts.empty()) ref_counts = cnt;
     else if (cnt != ref_counts)
@@ -1319,20 +1421,21 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
   }
   const float ms0 = best_ms;
   // A candidate is timed in three pairs interleaved with the incumbent (so a
-  // drift of the clocks hits both alike) and must beat it by 3% on the best
-  // of each.
+  // drift of the clocks hits both alike) and must beat it by 3% in every pair.
   auto consider = [&](const Cfg &c) -> int {
     float tb = 1e30f, tc = 1e30f;
+    bool wins = true;
     for (int r = 0; r < 3; r++) {
-      float t = 0;
+      float t = 0, u = 0;
       int rc2 = timed(best, t);
       if (rc2) return rc2;
       tb = std::min(tb, t);
-      if ((rc2 = timed(c, t))) return rc2;
-      tc = std::min(tc, t);
+      if ((rc2 = timed(c, u))) return rc2;
+      tc = std::min(tc, u);
+      wins = wins && u < 0.97f * t;
     }
     trials++;
-    if (tc < 0.97f * tb) {
+    if (wins) {
       best = c;
       best_ms = tc;
     } else {
@@ -1520,83 +1623,6 @@ void publish(dsm_server *sv, ServerReq *r, int status) {
 // negatives of R (if any); then P's count slot is copied to the host. Waits
 // for the round; while it runs, computes R's delta into *d. With `stats`, the
 // round's work counters and time go to the engine's statistics.
-template <typename T>
-void swap_buf(DevBuf<T> &a, DevBuf<T> &b) {
-  std::swap(a.p, b.p);
-  std::swap(a.n, b.n);
-}
-
-// Exchanges the engine's graph with the second copy.
-void swap_graph(dsm_engine *e) {
-  auto &c = e->shadow;
-  swap_buf(e->pool, c.pool);
-  swap_buf(e->start, c.start);
-  swap_buf(e->deg, c.deg);
-  swap_buf(e->cap, c.cap);
-  swap_buf(e->lpool, c.lpool);
-  swap_buf(e->loff, c.loff);
-  swap_buf(e->lab, c.lab);
-  swap_buf(e->bm, c.bm);
-  swap_buf(e->actl, c.actl);
-  std::swap(e->pool_cap, c.pool_cap);
-}
-
-template <typename T>
-hipError_t copy_buf(DevBuf<T> &dst, const DevBuf<T> &src, size_t count, hipStream_t s) {
-  if (dst.n < src.n) {
-    hipError_t err = dst.alloc(src.n);
-    if (err != hipSuccess) return err;
-  }
-  return count ? hipMemcpyAsync(dst.p, src.p, count * sizeof(T), hipMemcpyDeviceToDevice, s) : hipSuccess;
-}
-
-// Makes the engine's graph a copy of `c` on the engine's stream. The pool
-// and the label-sorted copies are copied up to the pool's used end (the
-// host's ApplyCtl mirror holds it after every update).
-int copy_graph(dsm_engine *e, const dsm_engine::GraphCopy &c) {
-  unsigned long long used = e->h_actl.p->pool_used;
-  if (used == 0 || used > c.pool_cap) used = c.pool_cap;
-  HIPCHK(copy_buf(e->pool, c.pool, used, e->stream));
-  HIPCHK(copy_buf(e->lpool, c.lpool, used, e->stream));
-  HIPCHK(copy_buf(e->start, c.start, c.start.n, e->stream));
-  HIPCHK(copy_buf(e->deg, c.deg, c.deg.n, e->stream));
-  HIPCHK(copy_buf(e->cap, c.cap, c.cap.n, e->stream));
-  HIPCHK(copy_buf(e->loff, c.loff, c.loff.n, e->stream));
-  HIPCHK(copy_buf(e->lab, c.lab, c.lab.n, e->stream));
-  HIPCHK(copy_buf(e->bm, c.bm, c.bm.n, e->stream));
-  HIPCHK(copy_buf(e->actl, c.actl, 1, e->stream));
-  e->pool_cap = c.pool_cap;
-  return DSM_OK;
-}
-
-// Applies one batch to the second copy of the graph while a round that was
-// enqueued on `stream` matches on the first: the copies swap, the new current
-// one becomes a copy of the old one (after ev_pre, i.e. after every update
-// enqueued before the round) and takes the batch, all on stream2. Returns
-// once the update is done; the copies stay swapped.
-int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats) {
-  std::swap(e->stream, e->stream2);
-  swap_graph(e);
-  int rc = DSM_OK;
-  auto run = [&]() -> int {
-    HIPCHK(hipStreamWaitEvent(e->stream, e->ev_pre, 0));
-    if (stats) HIPCHK(hipEventRecord(e->ev[0], e->stream));
-    if (int r = copy_graph(e, e->shadow)) return r;
-    if (int r = apply_delta_and_refresh(e, d)) return r;
-    if (stats) HIPCHK(hipEventRecord(e->ev[2], e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
-    if (stats) {
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
-      e->stats.ms_apply += ms;
-    }
-    return DSM_OK;
-  };
-  rc = run();
-  std::swap(e->stream, e->stream2);
-  return rc;
-}
-
 // One pipelined round: the positives of P and the negatives of R in one DFS
 // launch on the current graph. With `applied` given and overlap_apply on, R's
 // update is applied to the second copy of the graph meanwhile and *applied is
