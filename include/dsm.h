@@ -288,7 +288,9 @@ typedef struct dsm_file dsm_file;
  * id order, m `e a b` lines, then for data files `-` and batches `b bid k`
  * with k `a|d src dst label` lines, bid = 0,1,... in order. A v-line degree
  * that differs from the vertex's e-line count is DSM_EFORMAT (the reference
- * would build a corrupt CSR, io.cu:174). */
+ * would build a corrupt CSR, io.cu:174). A data file that ends after its e
+ * lines, without `-`, is the reference's static format (readGraphToCPU2,
+ * io.cu:27-120): a graph with no batches. */
 int dsm_file_read(const char *path, int is_query, dsm_file **out);
 void dsm_file_free(dsm_file *f);
 /* Views into the parsed file; valid until dsm_file_free. */

@@ -57,6 +57,21 @@ def test_parser_agrees_with_oracle(golden_dir, name):
     assert (qn, ql.tolist(), qu.tolist(), qv.tolist()) == (qo.n, qo.labels.tolist(), qo.eu.tolist(), qo.ev.tolist())
 
 
+def test_static_format_is_a_graph_without_batches(golden_dir, tmp_path):
+    """The static format (t/v/e lines only, io.cu:27-120) parses, in data and
+    in query mode, to the dynamic file's initial graph with no batches."""
+    src = open(os.path.join(golden_dir, "small_g.in")).read()
+    p = tmp_path / "static.in"
+    p.write_text(src[:src.index("-")])
+    want = E.DsmFile(os.path.join(golden_dir, "small_g.in"), False).graph()
+    for is_query in (False, True):
+        f = E.DsmFile(str(p), is_query)
+        n, lab, eu, ev = f.graph()
+        assert n == want[0] and lab.tolist() == want[1].tolist()
+        assert eu.tolist() == want[2].tolist() and ev.tolist() == want[3].tolist()
+        assert f.num_batches == 0
+
+
 BAD = {
     "no_header": "v 0 0 0\n",
     "degree_mismatch": "t 2 1\nv 0 0 2\nv 1 0 1\ne 0 1\n-\n",
