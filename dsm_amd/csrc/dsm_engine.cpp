@@ -156,6 +156,7 @@ struct dsm_engine {
   hipEvent_t ev_pre = nullptr;  // all work on stream before a pipelined round
   bool overlap_apply = true;
   bool lpool_stale = false;  // the pool grew during the current update
+  int cu_lds = 160 * 1024;   // LDS bytes per CU
   // per-wave task time of the last DFS launch (balance statistics)
   DevBuf<unsigned long long> wave_busy;
   PinnedBuf<unsigned long long> h_wave_busy;
@@ -993,6 +994,21 @@ int apply_and_refresh(dsm_engine *e, const dsm_change *changes, uint32_t k) {
   return apply_delta_and_refresh(e, d);
 }
 
+// Blocks per CU of a DFS shape: the occupancy limit and, with overlap_apply,
+// at most what leaves 32 KB of the CU's LDS to the update kernels that run
+// beside a pipelined round (k_apply_merge takes 16 KB per block). A shape
+// that fills the LDS makes them wait for the round.
+uint32_t dfs_bpc(const dsm_engine *e, uint32_t wpb) {
+  uint32_t b = (uint32_t)std::max(1, dfs_max_blocks_per_cu(wpb, e->max_qn));
+  if (e->overlap_apply) {
+    const uint32_t per_block = wpb * dfs_lds_bytes_per_wave(e->max_qn);
+    const int room = e->cu_lds - 32 * 1024;
+    const uint32_t cap = room > (int)per_block ? (uint32_t)room / per_block : 1u;
+    b = std::min(b, cap);
+  }
+  return b;
+}
+
 int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc *queries,
                 uint32_t num_queries) {
   int rc = 0;
@@ -1002,6 +1018,14 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   HIPCHK(hipEventCreateWithFlags(&e->ev_pre, hipEventDisableTiming));
   HIPCHK(hipDeviceGetAttribute(&e->num_cu, hipDeviceAttributeMultiprocessorCount, e->opt.device));
   HIPCHK(hipDeviceGetAttribute(&e->wall_khz, hipDeviceAttributeWallClockRate, e->opt.device));
+  {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, e->opt.device) ==
+            hipSuccess &&
+        lds > 0)
+      e->cu_lds = lds;
+  }
+  e->overlap_apply = e->opt.overlap_apply != 0;
   if (e->wall_khz <= 0) e->wall_khz = 100000;
   if (e->opt.watchdog_ms) e->watchdog_ms = e->opt.watchdog_ms;
   for (auto &x : e->ev) HIPCHK(hipEventCreate(&x));
@@ -1019,7 +1043,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   // keep a block's LDS (waves x per-wave stack) within 64 KB
   while (wpb > 1 && wpb * dfs_lds_bytes_per_wave(e->max_qn) > 65536) wpb /= 2;
   e->dfs_wpb = wpb;
-  int per_cu = dfs_max_blocks_per_cu(wpb, e->max_qn);
+  int per_cu = (int)dfs_bpc(e, wpb);
   if (e->opt.blocks_per_cu && (int)e->opt.blocks_per_cu < per_cu) per_cu = (int)e->opt.blocks_per_cu;
   e->dfs_blocks = (uint32_t)(per_cu * e->num_cu);
   // room for the largest grid any launch shape can have (the autotuner may
@@ -1039,7 +1063,6 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   e->graph_rounds = e->opt.graph_rounds != 0;
   e->leaf_segments = e->opt.leaf_segments != 0;
   e->poll_max = e->opt.poll_max ? std::min<uint32_t>(e->opt.poll_max, 16u) : 16u;
-  e->overlap_apply = e->opt.overlap_apply != 0;
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
   HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -1367,7 +1390,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     bool ls, seg;
     uint32_t dev;
   };
-  auto max_bpc = [&](uint32_t wpb) { return (uint32_t)std::max(1, dfs_max_blocks_per_cu(wpb, e->max_qn)); };
+  auto max_bpc = [&](uint32_t wpb) { return dfs_bpc(e, wpb); };
   // With overlap_apply, pipelined rounds run beside the next batch's update
   // (on the second copy of the graph), which competes for the CUs: each trial
   // round then runs beside the update of the first quarter of the sample (the
