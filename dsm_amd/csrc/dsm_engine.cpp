@@ -1025,16 +1025,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
       e->cu_lds = lds;
   }
   e->overlap_apply = e->opt.overlap_apply != 0;
-  if (e->opt.update_cus && e->opt.update_cus < (uint32_t)e->num_cu) {
-    // the overlapped update's stream runs on every (num_cu / update_cus)-th
-    // CU; the round keeps the others to itself
-    std::vector<uint32_t> mask(((uint32_t)e->num_cu + 31) / 32, 0u);
-    const uint32_t step = (uint32_t)e->num_cu / e->opt.update_cus;
-    for (uint32_t i = 0, c = 0; i < e->opt.update_cus; i++, c += step) mask[c >> 5] |= 1u << (c & 31);
-    HIPCHK(hipExtStreamCreateWithCUMask(&e->stream2, (uint32_t)mask.size(), mask.data()));
-  } else {
-    HIPCHK(hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking));
-  }
+  HIPCHK(hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking));
   if (e->wall_khz <= 0) e->wall_khz = 100000;
   if (e->opt.watchdog_ms) e->watchdog_ms = e->opt.watchdog_ms;
   for (auto &x : e->ev) HIPCHK(hipEventCreate(&x));

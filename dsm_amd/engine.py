@@ -68,7 +68,7 @@ class Options(C.Structure):
                 ("donate_min", C.c_uint32), ("queue_capacity", C.c_uint32), ("watchdog_ms", C.c_uint32),
                 ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("donate_every", C.c_uint32),
                 ("donate_min_n2", C.c_uint32), ("poll_max", C.c_uint32), ("leaf_segments", C.c_int),
-                ("graph_rounds", C.c_int), ("overlap_apply", C.c_int), ("update_cus", C.c_uint32)]
+                ("graph_rounds", C.c_int), ("overlap_apply", C.c_int)]
 
 
 class TuneReport(C.Structure):
@@ -250,8 +250,7 @@ class Engine:
     def __init__(self, graph, queries, device=0, relabel=True, count_negatives=True, tuple_capacity=0,
                  tuples_in_host_memory=False, waves_per_block=4, blocks_per_cu=0, donate_min=0,
                  queue_capacity=0, watchdog_ms=0, lane_work=0, leaf_sets=True, graph_rounds=True,
-                 donate_every=0, donate_min_n2=0, leaf_segments=True, poll_max=0, overlap_apply=True,
-                 update_cus=0):
+                 donate_every=0, donate_min_n2=0, leaf_segments=True, poll_max=0, overlap_apply=True):
         L = lib()
         n, lab, eu, ev = graph
         self._keep = []
@@ -283,7 +282,6 @@ class Engine:
         opt.leaf_segments = int(bool(leaf_segments))
         opt.poll_max = poll_max
         opt.overlap_apply = int(bool(overlap_apply))
-        opt.update_cus = update_cus
         h = C.c_void_p()
         rc = L.dsm_create(C.byref(gd), qds, len(queries), C.byref(opt), C.byref(h))
         self._keep = None

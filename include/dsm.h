@@ -99,9 +99,6 @@ typedef struct {
                                 graph while the round matches on the first, which
                                 doubles the graph's device memory; 0: apply after the
                                 round */
-  uint32_t update_cus;       /* with overlap_apply: the update's stream runs on this many
-                                CUs, spread evenly, so that the round keeps the others to
-                                itself (0 = all CUs) */
 } dsm_options;
 
 /* Per-batch statistics of the last dsm_apply_batch + dsm_match_batch. */
