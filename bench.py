@@ -80,6 +80,8 @@ def _agg_add(agg, st):
     agg["busy_ms"] += st["dfs_busy_ms"]
     agg["waves"] += st["dfs_waves"]
     agg["imbalance"] += st["dfs_block_imbalance"]
+    agg["head_idle"] += st["dfs_head_idle"]
+    agg["tail_idle"] += st["dfs_tail_idle"]
     agg["ms_apply"] += st["ms_apply"]
     agg["ms_filter"] += st["ms_filter"]
     agg["ms_seed"] += st["ms_seed"]
@@ -105,7 +107,7 @@ def run_batched(w, warmup, steps, engine_kw, autotune=True, pipelined=True):
         log("autotune %.2f s: %s" % (time.perf_counter() - t0, tuned))
     lat, recs = [], 0
     agg = {"ms_dfs": 0.0, "launches": 0, "entries": 0, "loads": 0, "merge_bytes": 0, "busy_ms": 0.0, "waves": 0,
-           "imbalance": 0.0, "ms_apply": 0.0, "ms_filter": 0.0, "ms_seed": 0.0, "seeds": 0, "donated": 0}
+           "imbalance": 0.0, "head_idle": 0.0, "tail_idle": 0.0, "ms_apply": 0.0, "ms_filter": 0.0, "ms_seed": 0.0, "seeds": 0, "donated": 0}
     pos_tot = neg_tot = 0
     per_batch = []
     try:
@@ -463,6 +465,8 @@ def main():
            # task time over the mean block's, per launch
            "dfs_balance": {"wave_utilization": round(agg["busy_ms"] / max(agg["waves"] / nl * agg["ms_dfs"], 1e-9), 4),
                            "block_imbalance": round(agg["imbalance"] / nl, 4),
+                           "head_idle": round(agg["head_idle"] / nl, 4),
+                           "tail_idle": round(agg["tail_idle"] / nl, 4),
                            "waves_per_launch": int(agg["waves"] / nl)}}
 
     if not args.no_plain:

@@ -663,7 +663,7 @@ int graph_round(dsm_engine *e, const RoundPart *parts, int nparts, uint32_t nrec
     HIPCHK(hipGraphAddMemcpyNode1D(&cp, e->rg_graph, &e->rg_dfs, 1, e->h_ctl.p, e->ctl.p, sizeof(DfsCtl),
                                    hipMemcpyDeviceToHost));
     HIPCHK(hipGraphAddMemcpyNode1D(&cw, e->rg_graph, &cp, 1, e->h_wave_busy.p, e->wave_busy.p,
-                                   (size_t)e->wave_slots * 8, hipMemcpyDeviceToHost));
+                                   (size_t)e->wave_slots * 24, hipMemcpyDeviceToHost));
     HIPCHK(hipGraphInstantiate(&e->rg_exec, e->rg_graph, nullptr, nullptr, 0));
     e->rg_key = key;
   } else {
@@ -694,7 +694,7 @@ int dfs_round(dsm_engine *e, const RoundPart *parts, int nparts, hipEvent_t ev0,
     if (ev1) HIPCHK(hipEventRecord(ev1, e->stream));
     HIPCHK(launch_dfs(dfs_args(e), dfs_grid(e, nrec), e->dfs_wpb, e->max_qn, e->stream));
     HIPCHK(hipMemcpyAsync(e->h_ctl.p, e->ctl.p, sizeof(DfsCtl), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(e->h_wave_busy.p, e->wave_busy.p, (size_t)e->wave_slots * 8, hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(e->h_wave_busy.p, e->wave_busy.p, (size_t)e->wave_slots * 24, hipMemcpyDeviceToHost,
                           e->stream));
   }
   if (ev2) HIPCHK(hipEventRecord(ev2, e->stream));
@@ -734,6 +734,30 @@ int check_round(dsm_engine *e, uint64_t *seeds) {
       e->stats.dfs_busy_ms += tot * to_ms;
       e->stats.dfs_waves += (uint64_t)nb * wpb;
       if (tot > 0) e->stats.dfs_block_imbalance += mx / (tot / nb);
+      // head and tail idle: a wave without tasks counts as idle in the tail
+      const unsigned long long *first = e->h_wave_busy.p + e->wave_slots;
+      const unsigned long long *last = first + e->wave_slots;
+      const uint32_t nw = nb * wpb;
+      unsigned long long t0 = ~0ull, t1 = 0;
+      for (uint32_t w = 0; w < nw; w++)
+        if (first[w]) {
+          t0 = std::min(t0, first[w]);
+          t1 = std::max(t1, last[w]);
+        }
+      if (t1 > t0) {
+        const double span = (double)(t1 - t0);
+        double head = 0, tail = 0;
+        for (uint32_t w = 0; w < nw; w++) {
+          if (!first[w]) {
+            tail += span;
+            continue;
+          }
+          head += (double)(first[w] - t0);
+          tail += (double)(t1 - last[w]);
+        }
+        e->stats.dfs_head_idle += head / (span * nw);
+        e->stats.dfs_tail_idle += tail / (span * nw);
+      }
     }
   }
   e->stats.dfs_launches += 1;
@@ -1052,8 +1076,9 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
     uint32_t most = 0;
     for (uint32_t w : {1u, 2u, 4u, 8u}) most = std::max<uint32_t>(most, (uint32_t)dfs_max_blocks_per_cu(w, e->max_qn) * w);
     e->wave_slots = most * (uint32_t)e->num_cu;
-    HIPCHK(e->wave_busy.alloc(e->wave_slots));
-    HIPCHK(e->h_wave_busy.ensure(e->wave_slots));
+    // per wave: task time, first task start, last task end
+    HIPCHK(e->wave_busy.alloc(3 * (size_t)e->wave_slots));
+    HIPCHK(e->h_wave_busy.ensure(3 * (size_t)e->wave_slots));
   }
   e->donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
   e->lane_work = e->opt.lane_work ? e->opt.lane_work : 16;

@@ -943,6 +943,12 @@ struct WaveHdr {
 };
 constexpr uint32_t kWin = 256;   // window entries staged in LDS per intersection
 constexpr uint32_t kFCap = 2048;  // largest leaf set F kept in LDS
+// One level above the leaf, the candidates of a 64-entry window count their
+// leaf images inline until their summed scan or search work passes this many
+// entries; the candidate that passes it descends to the leaf level instead,
+// where the scan can be split with idle waves (an inline count cannot), and
+// the window resumes after it.
+constexpr uint32_t kLeafBudget = 2048;
 // states of the leaf set F
 constexpr uint32_t kFStale = 0;  // not computed for the current positions
 constexpr uint32_t kFReady = 1;  // fbuf[0, fn) holds F
@@ -1477,6 +1483,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
   uint32_t naps = 0;            // polls of the current ticket (backoff)
   uint64_t idle_since = 0;      // wall clock when the wave ran out of work (watchdog)
   uint64_t busy = 0;            // wall-clock ticks spent on tasks
+  uint64_t first_t = 0, last_t = 0;  // start of the first task, end of the last
   for (;;) {
     // lane 0 claims work; every decision is then broadcast so that all
     // control flow below stays wave-uniform
@@ -1535,6 +1542,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
     }
     // ---- load the task
     const uint64_t t_task = wall_clock64();
+    if (first_t == 0) first_t = t_task;
     uint32_t plan, L0, lo, hi, tag, fw = 0;
     if (got == 1) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1647,7 +1655,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
           // one level above the leaf: each lane counts the leaf images of its
           // own candidate against the shared leaf set F, instead of a descent
           // per candidate
-          bool large = false;
+          bool large = false, descend = false;
           for (;;) {
             if (++tick >= a.donate_every) {
               tick = 0;
@@ -1675,17 +1683,52 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
             }
             bool defer = false, scan = false;
             LeafList ll{nullptr, 0, false};
-            uint32_t wdeg = 0;
-            if (ok) {
-              lcount += leaf_count_lane(M, a, l + 1, w, defer, scan, ll, scanned, loads, wdeg);
+            uint32_t wdeg = 0, lc = 0;
+            if (ok) lc = leaf_count_lane(M, a, l + 1, w, defer, scan, ll, scanned, loads, wdeg);
+            // inline work of the lanes up to this one; the first lane past the
+            // budget descends, the lanes after it are tested again later
+            uint32_t wk = 0;
+            if (scan || defer) wk = ll.by_f ? S.fn * (33u - (uint32_t)__builtin_clz(ll.len | 1u)) : ll.len;
+            uint32_t incl = wk;
+            for (int o = 1; o < 64; o <<= 1) {
+              const uint32_t y = __shfl_up(incl, o);
+              if (lane >= (uint32_t)o) incl += y;
+            }
+            const uint64_t dm = ballot(wk > 0 && incl > kLeafBudget);
+            const uint32_t k0 = dm ? (uint32_t)__builtin_ctzll(dm) : 64u;
+            if (lane >= k0) {
+              lc = 0;
+              scan = defer = false;
+            } else if (ok) {
               mbytes += fixed_leaf_bytes + 4ull * wdeg;
             }
+            lcount += lc;
             if (a.leaf_segments) lcount += leaf_scan_segments(M, a, l + 1, w, scan, ll, scanned);
             lcount += leaf_count_wave(M, a, l + 1, w, defer, ll, scanned, loads);
+            if (dm) {
+              // lane k0's candidate as a one-entry chunk at source index p + k0
+              const uint32_t w0 = __shfl(w, (int)k0);
+              wave_sync();
+              if (lane == 0) {
+                const uint32_t c = (l - 2) * 64;
+                M.cand[c] = w0;
+                M.cst[c] = a.g.start[w0];
+                M.cdg[c] = a.g.deg[w0];
+                S.cn[l] = 1;
+                S.ci[l] = 0;
+                S.cbase[l] = p + k0;
+                S.cmask[l] = 1;
+                S.pos[l] = p + k0 + 1;
+              }
+              wave_sync();
+              descend = true;
+              break;
+            }
             wave_sync();
             if (lane == 0) S.pos[l] = p + 64;
             wave_sync();
           }
+          if (descend) continue;
           if (!large) {
             if (l == L0) break;
             l--;
@@ -1747,11 +1790,17 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
       atomicAdd(&a.ctl->mbytes.v, (unsigned long long)mb);
       atomicSub(&a.ctl->work_left.v, 1);
     }
-    busy += wall_clock64() - t_task;
+    last_t = wall_clock64();
+    busy += last_t - t_task;
   }
-  // time this wave spent on tasks (not waiting), for the balance statistics
+  // time this wave spent on tasks (not waiting) and when its first task
+  // started and its last ended, for the balance statistics
   const uint32_t slot_w = blockIdx.x * WPB + (threadIdx.x >> 6);
-  if (a.wave_busy && lane == 0 && slot_w < a.wave_slots) a.wave_busy[slot_w] = busy;
+  if (a.wave_busy && lane == 0 && slot_w < a.wave_slots) {
+    a.wave_busy[slot_w] = busy;
+    a.wave_busy[a.wave_slots + slot_w] = first_t;
+    a.wave_busy[2 * (size_t)a.wave_slots + slot_w] = last_t;
+  }
 }
 
 // Copies every vertex's live list to its new slot (pool growth/compaction).

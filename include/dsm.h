@@ -125,6 +125,11 @@ typedef struct {
   double dfs_busy_ms;
   uint64_t dfs_waves;
   double dfs_block_imbalance;
+  /* where the waves' idle time lies, summed over the launches as fractions of
+   * waves x (last task end - first task start): before a wave's first task,
+   * and after its last one (the tail) */
+  double dfs_head_idle;
+  double dfs_tail_idle;
 } dsm_stats;
 
 /* Outcome of dsm_autotune: the launch parameters kept and the DFS time of the
