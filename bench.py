@@ -123,9 +123,12 @@ def run_batched(w, warmup, steps, engine_kw, autotune=True, pipelined=True):
             log("warmup batch %d: %.2f ms" % (b, (time.perf_counter() - t) * 1e3))
         timed = [w.batch(b) for b in range(warmup, warmup + steps)]
         recs = sum(len(x[1]) for x in timed)
+        qms = None
         if pipelined:
+            eng.query_time()
             t0m = time.monotonic()
             pos, neg, done = eng.process_batches(timed)
+            qms = eng.query_time()
             prev = t0m
             for b in range(steps):
                 lat.append(done[b] - prev)
@@ -155,7 +158,18 @@ def run_batched(w, warmup, steps, engine_kw, autotune=True, pipelined=True):
     finally:
         eng.close()
     return {"lat": lat, "records": recs, "pos": pos_tot, "neg": neg_tot, "agg": agg, "per_batch": per_batch,
-            "tuned": tuned}
+            "tuned": tuned, "query_ms": qms}
+
+
+def query_share(w, qms, top=8):
+    """The queries with the most DFS task time in the timed region, with their
+    share of it."""
+    if qms is None or qms.sum() <= 0:
+        return None
+    order = np.argsort(-qms)[:top]
+    return [{"query": int(q), "vertices": int(w.queries[q].n), "edges": int(len(w.queries[q].eu)),
+             "shape": getattr(w.queries[q], "shape", ""), "share": round(float(qms[q] / qms.sum()), 4)}
+            for q in order]
 
 
 def run_plain(w, warmup, steps, engine_kw):
@@ -457,6 +471,7 @@ def main():
                                   "ms_dfs_per_batch": round(one["agg"]["ms_dfs"] / len(one["lat"]), 3)}
            if one else None,
            "matches_per_s": round((bat["pos"] + bat["neg"]) / total_s, 1),
+           "dfs_query_share": query_share(w, bat["query_ms"]),
            "stages_ms_per_batch": {k: round(agg[k] / len(bat["lat"]), 3) for k in ("ms_apply", "ms_filter",
                                                                                "ms_seed", "ms_dfs")},
            "roofline": roofline,

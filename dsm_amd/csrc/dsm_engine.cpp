@@ -160,6 +160,7 @@ struct dsm_engine {
   // per-wave task time of the last DFS launch (balance statistics)
   DevBuf<unsigned long long> wave_busy;
   PinnedBuf<unsigned long long> h_wave_busy;
+  DevBuf<unsigned long long> query_busy;  // per query: DFS task ticks since the last read
   uint32_t wave_slots = 0, last_blocks = 0, last_wpb = 0;
   // the DFS round as a HIP graph (graph_round)
   hipGraph_t rg_graph = nullptr;
@@ -519,6 +520,8 @@ int upload_tables(dsm_engine *e) {
   HIPCHK(hipMemcpy(e->d_plan_key.p, e->plan_key.data(), e->plan_key.size() * 4, hipMemcpyHostToDevice));
   const uint32_t nq = (uint32_t)e->queries.size();
   HIPCHK(e->counts.alloc((size_t)nq * 2 * kMaxSlots));
+  HIPCHK(e->query_busy.alloc(nq));
+  HIPCHK(hipMemset(e->query_busy.p, 0, (size_t)nq * 8));
   HIPCHK(e->h_counts.ensure((size_t)nq * 2 * kMaxSlots));
   HIPCHK(e->ctl.alloc(1));
   HIPCHK(e->h_ctl.ensure(1));
@@ -616,6 +619,7 @@ DfsArgs dfs_args(dsm_engine *e) {
   da.tuple_cap = e->tuple_cap;
   da.tuple_count = e->tuple_count.p;
   da.wave_busy = e->wave_busy.p;
+  da.query_busy = e->query_busy.p;
   da.wave_slots = e->wave_slots;
   return da;
 }
@@ -1388,6 +1392,17 @@ int dsm_get_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words) {
   HIPCHK(hipStreamSynchronize(e->stream));
   const uint32_t s = e->qsig[q][u];
   HIPCHK(hipMemcpy(words, e->bm.p + (size_t)s * e->words, (size_t)e->words * 8, hipMemcpyDeviceToHost));
+  return DSM_OK;
+}
+
+int dsm_get_query_time(dsm_engine *e, double *ms, uint32_t num_queries) {
+  if (!e || !ms || num_queries != e->queries.size()) return DSM_EINVAL;
+  HIPCHK(hipSetDevice(e->opt.device));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  std::vector<unsigned long long> t(num_queries);
+  HIPCHK(hipMemcpy(t.data(), e->query_busy.p, (size_t)num_queries * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemset(e->query_busy.p, 0, (size_t)num_queries * 8));
+  for (uint32_t q = 0; q < num_queries; q++) ms[q] = (double)t[q] / (double)e->wall_khz;
   return DSM_OK;
 }
 

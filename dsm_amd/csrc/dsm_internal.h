@@ -168,6 +168,7 @@ struct DfsArgs {
   unsigned long long *tuple_count;  // tuples found this batch
   unsigned long long *wave_busy;    // per wave of the grid: wall-clock ticks on tasks (or null)
   uint32_t wave_slots;              // entries of wave_busy
+  unsigned long long *query_busy;   // per query: wall-clock ticks of its tasks (or null)
 };
 
 struct SeedArgs {

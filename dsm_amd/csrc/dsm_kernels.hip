@@ -1792,6 +1792,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
     }
     last_t = wall_clock64();
     busy += last_t - t_task;
+    if (a.query_busy && lane == 0) atomicAdd(&a.query_busy[S.query], (unsigned long long)(last_t - t_task));
   }
   // time this wave spent on tasks (not waiting) and when its first task
   // started and its last ended, for the balance statistics

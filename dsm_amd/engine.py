@@ -34,7 +34,7 @@ TUPLE_WORDS = MAX_QUERY_VERTICES + 2
 EXPORTS = (
     "dsm_options_default", "dsm_create", "dsm_destroy", "dsm_last_error", "dsm_apply_batch",
     "dsm_match_batch", "dsm_process_batch", "dsm_get_tuples", "dsm_get_stats", "dsm_get_graph",
-    "dsm_get_candidates", "dsm_num_vertices", "dsm_num_queries", "dsm_file_read", "dsm_file_free",
+    "dsm_get_candidates", "dsm_get_query_time", "dsm_num_vertices", "dsm_num_queries", "dsm_file_read", "dsm_file_free",
     "dsm_file_graph", "dsm_file_query", "dsm_file_num_batches", "dsm_file_batch", "dsm_file_error",
     "dsm_server_create", "dsm_server_submit", "dsm_server_wait", "dsm_server_error", "dsm_server_destroy",
     "dsm_admit", "dsm_estimate_request_bytes", "dsm_autotune", "dsm_process_batches",
@@ -145,6 +145,8 @@ def lib():
     L.dsm_get_graph.argtypes = [vp, vp, vp, C.c_uint64, vp, C.POINTER(C.c_uint64)]
     L.dsm_get_candidates.restype = C.c_int
     L.dsm_get_candidates.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
+    L.dsm_get_query_time.restype = C.c_int
+    L.dsm_get_query_time.argtypes = [vp, vp, C.c_uint32]
     L.dsm_num_vertices.restype = C.c_uint32
     L.dsm_num_vertices.argtypes = [vp]
     L.dsm_num_queries.restype = C.c_uint32
@@ -386,6 +388,12 @@ class Engine:
         self._check(lib().dsm_get_candidates(self._h, q, u, words.ctypes.data))
         bits = np.unpackbits(words.view(np.uint8), bitorder="little")
         return bits[: self.n].astype(bool)
+
+    def query_time(self):
+        """DFS task time per query in ms since the last call (resets it)."""
+        ms = np.zeros(self.nq, np.float64)
+        self._check(lib().dsm_get_query_time(self._h, ms.ctypes.data, self.nq))
+        return ms
 
 
 def admit(request_bytes, in_flight_bytes, budget):

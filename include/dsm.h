@@ -221,6 +221,12 @@ int dsm_get_graph(dsm_engine *e, uint64_t *offsets, uint32_t *neighbours, uint64
  * reference keeps the equivalent in d_compact_encodings_ (main.cu:141-142). */
 int dsm_get_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words);
 
+/* DFS task time per query (ms, summed over the waves, so it can exceed the
+ * wall time) since the engine was created or this was last called; the call
+ * resets it. No counterpart in the reference, which runs one query per
+ * process; it tells which standing queries cost the most. */
+int dsm_get_query_time(dsm_engine *e, double *ms, uint32_t num_queries);
+
 /* Number of vertices and queries of the engine. */
 uint32_t dsm_num_vertices(const dsm_engine *e);
 uint32_t dsm_num_queries(const dsm_engine *e);

@@ -560,3 +560,24 @@ def test_process_batches_grows_pool(overlap):
         assert off.tolist() == ooff.tolist() and np.array_equal(nb, onb)
     finally:
         eng.close()
+
+
+def test_query_time_reports_and_resets():
+    """dsm_get_query_time: per-query DFS task time since the last call; a
+    query that seeds nothing costs nothing, and the call resets the sums."""
+    say("case: per-query DFS time")
+    rng = np.random.default_rng(1919)
+    n, m, nl = 400, 4000, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    # the third query's label 7 exists nowhere in the graph: no seeds
+    queries = [qtuple("triangle", [0, 1, 0]), qtuple("square", [0, 1, 0, 1]), qtuple("triangle", [7, 7, 7])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    eng = E.Engine((n, labels, eu, ev), queries, relabel=False)
+    try:
+        eng.query_time()
+        eng.process_batch(*conflict_free_batch(n, 200, present, rng))
+        t = eng.query_time()
+        assert t.shape == (3,) and t[0] > 0 and t[1] > 0 and t[2] == 0
+        assert np.all(eng.query_time() == 0)
+    finally:
+        eng.close()
