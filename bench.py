@@ -394,6 +394,7 @@ def main():
     ap.add_argument("--donate-min-n2", type=int, default=0)
     ap.add_argument("--leaf-segments", type=int, default=1)
     ap.add_argument("--poll-max", type=int, default=0)
+    ap.add_argument("--leaf-budget", type=int, default=0, help="0: the engine's default (2048)")
     ap.add_argument("--overlap-apply", type=int, default=1,
                     help="1: pipelined batches update a second copy of the graph during the round")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run one D2D copy of this many bytes")
@@ -414,7 +415,8 @@ def main():
                  "blocks_per_cu": args.blocks_per_cu, "lane_work": args.lane_work,
                  "leaf_sets": bool(args.leaf_sets), "donate_every": args.donate_every,
                  "donate_min_n2": args.donate_min_n2, "leaf_segments": bool(args.leaf_segments),
-                 "poll_max": args.poll_max, "overlap_apply": bool(args.overlap_apply)}
+                 "poll_max": args.poll_max, "overlap_apply": bool(args.overlap_apply),
+                 "leaf_budget": args.leaf_budget}
 
     batch = int(w.bstart[1] - w.bstart[0])
     # batches one at a time have no update to overlap: no second copy of the graph

@@ -136,7 +136,7 @@ struct dsm_engine {
   bool tuples_host = false;
   unsigned long long tuple_cap = 0;
   uint32_t dfs_blocks = 0, dfs_wpb = 4, max_qn = 2;
-  uint32_t donate_min = 256, lane_work = 16, donate_every = 8, donate_min_n2 = 64;
+  uint32_t donate_min = 256, lane_work = 16, donate_every = 8, donate_min_n2 = 64, leaf_budget = 2048;
   bool leaf_sets = true;
   bool graph_rounds = true;
   PinnedBuf<uint32_t> pipe_h_rec;  // records of a pipelined round (dsm_process_batches)
@@ -606,6 +606,7 @@ DfsArgs dfs_args(dsm_engine *e) {
   da.epoch = ++e->epoch;
   da.donate_min = e->donate_min;
   da.lane_work = e->lane_work;
+  da.leaf_budget = e->leaf_budget;
   da.donate_every = e->donate_every;
   da.donate_min_n2 = e->donate_min_n2;
   da.leaf_segments = e->leaf_segments ? 1u : 0u;
@@ -1086,6 +1087,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   }
   e->donate_min = e->opt.donate_min ? e->opt.donate_min : 256;
   e->lane_work = e->opt.lane_work ? e->opt.lane_work : 16;
+  e->leaf_budget = e->opt.leaf_budget ? e->opt.leaf_budget : 2048;
   e->donate_every = e->opt.donate_every ? e->opt.donate_every : 8;
   e->donate_min_n2 = e->opt.donate_min_n2 ? e->opt.donate_min_n2 : 64;
   e->leaf_sets = e->opt.leaf_sets != 0;
@@ -1126,6 +1128,25 @@ hipError_t copy_buf(DevBuf<T> &dst, const DevBuf<T> &src, size_t count, hipStrea
     if (err != hipSuccess) return err;
   }
   return count ? hipMemcpyAsync(dst.p, src.p, count * sizeof(T), hipMemcpyDeviceToDevice, s) : hipSuccess;
+}
+
+// Allocates the second copy of the graph at the current sizes; its contents
+// are written by the first overlapped update.
+int reserve_shadow(dsm_engine *e) {
+  auto &c = e->shadow;
+  auto like = [](auto &dst, const auto &src) -> hipError_t {
+    return dst.n < src.n ? dst.alloc(src.n) : hipSuccess;
+  };
+  HIPCHK(like(c.pool, e->pool));
+  HIPCHK(like(c.lpool, e->lpool));
+  HIPCHK(like(c.start, e->start));
+  HIPCHK(like(c.deg, e->deg));
+  HIPCHK(like(c.cap, e->cap));
+  HIPCHK(like(c.loff, e->loff));
+  HIPCHK(like(c.lab, e->lab));
+  HIPCHK(like(c.bm, e->bm));
+  HIPCHK(like(c.actl, e->actl));
+  return DSM_OK;
 }
 
 // Makes the engine's graph a copy of `c` on the engine's stream. The pool
@@ -1428,7 +1449,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
   struct Cfg {
     uint32_t wpb, bpc, dmin, lw;
     bool ls, seg;
-    uint32_t dev;
+    uint32_t dev, lb;
   };
   auto max_bpc = [&](uint32_t wpb) { return dfs_bpc(e, wpb); };
   // With overlap_apply, pipelined rounds run beside the next batch's update
@@ -1447,6 +1468,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     e->dfs_blocks = c.bpc * (uint32_t)e->num_cu;
     e->donate_min = c.dmin;
     e->lane_work = c.lw;
+    e->leaf_budget = c.lb;
     e->leaf_sets = c.ls;
     e->leaf_segments = c.seg;
     e->donate_every = c.dev;
@@ -1474,7 +1496,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     return DSM_OK;
   };
   Cfg best{e->dfs_wpb, e->dfs_blocks / (uint32_t)e->num_cu, e->donate_min, e->lane_work, e->leaf_sets,
-           e->leaf_segments, e->donate_every};
+           e->leaf_segments, e->donate_every, e->leaf_budget};
   // warm-up: clocks and caches settle before anything is compared
   float best_ms = 1e30f;
   for (int r = 0; r < 4; r++) {
@@ -1561,6 +1583,14 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
         c2.lw = v;
         if ((rc = consider(c2))) return rc;
       }
+      // 6. leaf budget per window (0xFFFFFFFF: none)
+      const Cfg base6 = best;
+      for (uint32_t v : {512u, 2048u, 8192u, 32768u, 0xFFFFFFFFu}) {
+        if (v == base6.lb) continue;
+        Cfg c6 = base6;
+        c6.lb = v;
+        if ((rc = consider(c6))) return rc;
+      }
     }
   }
   // the kept configuration, timed once more
@@ -1578,6 +1608,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     rep->leaf_sets = best.ls ? 1 : 0;
     rep->leaf_segments = best.seg ? 1 : 0;
     rep->donate_every = best.dev;
+    rep->leaf_budget = best.lb;
     rep->trials = trials;
     rep->ms_initial = ms0;
     rep->ms_best = best_ms;
@@ -1835,6 +1866,9 @@ int dsm_server_create(dsm_engine *e, const dsm_server_options *opt, dsm_server *
   if (e->tuple_cap) return e->fail_msg(DSM_EINVAL, "served mode keeps counts only (tuple_capacity must be 0)");
   HIPCHK(hipSetDevice(e->opt.device));
   HIPCHK(hipStreamSynchronize(e->stream));
+  // the second copy of the graph is part of the engine, not of the budget
+  if (e->overlap_apply)
+    if (int rc = reserve_shadow(e)) return rc;
   std::unique_ptr<dsm_server> sv(new dsm_server());
   sv->e = e;
   size_t freeb = 0, totalb = 0;

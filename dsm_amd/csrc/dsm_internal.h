@@ -153,6 +153,7 @@ struct DfsArgs {
   uint32_t epoch;
   uint32_t donate_min;
   uint32_t lane_work;            // longest leaf list one lane scans alone
+  uint32_t leaf_budget;          // inline leaf-count work of a window before a candidate descends
   uint32_t donate_every;         // chunks between checks for waiting waves
   uint32_t donate_min_n2;        // smallest range split off one level above the leaf
   uint32_t leaf_segments;        // leaf lists scanned as one segmented array

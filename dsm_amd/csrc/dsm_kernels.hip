@@ -944,11 +944,10 @@ struct WaveHdr {
 constexpr uint32_t kWin = 256;   // window entries staged in LDS per intersection
 constexpr uint32_t kFCap = 2048;  // largest leaf set F kept in LDS
 // One level above the leaf, the candidates of a 64-entry window count their
-// leaf images inline until their summed scan or search work passes this many
-// entries; the candidate that passes it descends to the leaf level instead,
-// where the scan can be split with idle waves (an inline count cannot), and
-// the window resumes after it.
-constexpr uint32_t kLeafBudget = 2048;
+// leaf images inline until their summed scan or search work passes
+// DfsArgs::leaf_budget entries; the candidate that passes it descends to the
+// leaf level instead, where the scan can be split with idle waves (an inline
+// count cannot), and the window resumes after it.
 // states of the leaf set F
 constexpr uint32_t kFStale = 0;  // not computed for the current positions
 constexpr uint32_t kFReady = 1;  // fbuf[0, fn) holds F
@@ -1694,7 +1693,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
               const uint32_t y = __shfl_up(incl, o);
               if (lane >= (uint32_t)o) incl += y;
             }
-            const uint64_t dm = ballot(wk > 0 && incl > kLeafBudget);
+            const uint64_t dm = ballot(wk > 0 && incl > a.leaf_budget);
             const uint32_t k0 = dm ? (uint32_t)__builtin_ctzll(dm) : 64u;
             if (lane >= k0) {
               lc = 0;

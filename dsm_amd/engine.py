@@ -68,14 +68,14 @@ class Options(C.Structure):
                 ("donate_min", C.c_uint32), ("queue_capacity", C.c_uint32), ("watchdog_ms", C.c_uint32),
                 ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("donate_every", C.c_uint32),
                 ("donate_min_n2", C.c_uint32), ("poll_max", C.c_uint32), ("leaf_segments", C.c_int),
-                ("graph_rounds", C.c_int), ("overlap_apply", C.c_int)]
+                ("graph_rounds", C.c_int), ("overlap_apply", C.c_int), ("leaf_budget", C.c_uint32)]
 
 
 class TuneReport(C.Structure):
     _fields_ = [("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("donate_min", C.c_uint32),
                 ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("leaf_segments", C.c_int),
                 ("donate_every", C.c_uint32), ("trials", C.c_uint32),
-                ("ms_initial", C.c_float), ("ms_best", C.c_float)]
+                ("ms_initial", C.c_float), ("ms_best", C.c_float), ("leaf_budget", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -252,7 +252,8 @@ class Engine:
     def __init__(self, graph, queries, device=0, relabel=True, count_negatives=True, tuple_capacity=0,
                  tuples_in_host_memory=False, waves_per_block=4, blocks_per_cu=0, donate_min=0,
                  queue_capacity=0, watchdog_ms=0, lane_work=0, leaf_sets=True, graph_rounds=True,
-                 donate_every=0, donate_min_n2=0, leaf_segments=True, poll_max=0, overlap_apply=True):
+                 donate_every=0, donate_min_n2=0, leaf_segments=True, poll_max=0, overlap_apply=True,
+                 leaf_budget=0):
         L = lib()
         n, lab, eu, ev = graph
         self._keep = []
@@ -284,6 +285,7 @@ class Engine:
         opt.leaf_segments = int(bool(leaf_segments))
         opt.poll_max = poll_max
         opt.overlap_apply = int(bool(overlap_apply))
+        opt.leaf_budget = leaf_budget
         h = C.c_void_p()
         rc = L.dsm_create(C.byref(gd), qds, len(queries), C.byref(opt), C.byref(h))
         self._keep = None

@@ -99,6 +99,10 @@ typedef struct {
                                 graph while the round matches on the first, which
                                 doubles the graph's device memory; 0: apply after the
                                 round */
+  uint32_t leaf_budget;      /* one level above the leaf: inline leaf-count work (list
+                                entries) of a 64-candidate window before a candidate
+                                descends so that its scan can be shared with idle waves
+                                (0 = 2048; 0xFFFFFFFF = never) */
 } dsm_options;
 
 /* Per-batch statistics of the last dsm_apply_batch + dsm_match_batch. */
@@ -140,6 +144,7 @@ typedef struct {
   uint32_t donate_every;
   uint32_t trials;        /* configurations timed */
   float ms_initial, ms_best;
+  uint32_t leaf_budget;
 } dsm_tune_report;
 
 /* ------------------------------------------------------------ engine life */

@@ -371,9 +371,12 @@ def test_autotune_keeps_counts_and_state():
         eng.close()
 
 
-@pytest.mark.parametrize("leaf_sets,lane_work", [(False, 0), (True, 1), (True, 1000)])
-def test_leaf_level_strategies_agree(leaf_sets, lane_work):
-    say("case: leaf level leaf_sets=%s lane_work=%d" % (leaf_sets, lane_work))
+@pytest.mark.parametrize("leaf_sets,lane_work,budget", [(False, 0, 0), (True, 1, 0), (True, 1000, 0),
+                                                        (True, 0, 16), (True, 0, 0xFFFFFFFF)])
+def test_leaf_level_strategies_agree(leaf_sets, lane_work, budget):
+    """Leaf sets off and on, lane work limits, and leaf budgets per window
+    (16: candidates descend constantly; 0xFFFFFFFF: never) against the oracle."""
+    say("case: leaf level leaf_sets=%s lane_work=%d budget=%d" % (leaf_sets, lane_work, budget))
     rng = np.random.default_rng(1212)
     n, m, nl = 300, 4000, 2
     labels, eu, ev = random_graph(n, m, nl, rng)
@@ -381,7 +384,8 @@ def test_leaf_level_strategies_agree(leaf_sets, lane_work):
                qtuple("star4", [0, 1, 1, 0, 1]), qtuple("cycle6_chord", [0, 1, 0, 1, 0, 1])]
     present = set(zip(eu.tolist(), ev.tolist()))
     batches = [conflict_free_batch(n, 60, present, rng) for _ in range(2)]
-    compare_stream(n, labels, eu, ev, queries, batches, relabel=True, leaf_sets=leaf_sets, lane_work=lane_work)
+    compare_stream(n, labels, eu, ev, queries, batches, relabel=True, leaf_sets=leaf_sets, lane_work=lane_work,
+                   leaf_budget=budget)
 
 
 @pytest.mark.parametrize("tuples", [False, True])
