@@ -196,9 +196,12 @@ int dsm_process_batch(dsm_engine *e, const dsm_change *changes, uint32_t num_cha
  * Batch b is changes[batch_offsets[b] .. batch_offsets[b+1]); the counts go
  * to pos_counts / neg_counts [num_batches][num_queries]; done_s (optional)
  * receives the steady-clock second at which each batch's counts were ready.
- * Counts only (tuple_capacity must be 0). The statistics cover the whole
- * call. No counterpart in the reference, which matches one batch per loop
- * iteration (main.cu:152-348). */
+ * With overlap_apply, batch b+1's update is applied to the second copy of
+ * the graph during that launch. Counts only (tuple_capacity must be 0). The
+ * statistics cover the whole call. DSM_EINVAL (bad offsets or endpoints) is
+ * returned before anything runs; after any other error the graph may be
+ * partly updated and the engine should be destroyed. No counterpart in the
+ * reference, which matches one batch per loop iteration (main.cu:152-348). */
 int dsm_process_batches(dsm_engine *e, const dsm_change *changes, const uint32_t *batch_offsets,
                         uint32_t num_batches, uint64_t *pos_counts, uint64_t *neg_counts, double *done_s);
 
