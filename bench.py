@@ -419,9 +419,7 @@ def main():
                  "leaf_budget": args.leaf_budget}
 
     batch = int(w.bstart[1] - w.bstart[0])
-    # batches one at a time have no update to overlap: no second copy of the graph
-    one = None if args.no_one_by_one else run_batched(w, args.warmup, args.steps,
-                                                      dict(engine_kw, overlap_apply=False),
+    one = None if args.no_one_by_one else run_batched(w, args.warmup, args.steps, engine_kw,
                                                       autotune=not args.no_autotune, pipelined=False)
     bat = run_batched(w, args.warmup, args.steps, engine_kw, autotune=not args.no_autotune, pipelined=True)
     for b, ((p1, n1), (p2, n2)) in enumerate(zip(one["per_batch"] if one else [], bat["per_batch"])):

@@ -1290,28 +1290,46 @@ int dsm_apply_batch(dsm_engine *e, const dsm_change *changes, uint32_t k) {
   const uint32_t nq = (uint32_t)e->queries.size();
   HIPCHK(launch_ctl_reset(nullptr, nullptr, e->tuple_count.p, e->counts.p, nq * 2, e->stream));
   int rc;
-  // 1. negatives in the pre-batch graph
+  bool applied = false;
+  // 1. negatives in the pre-batch graph; with overlap_apply the batch is
+  // applied to the second copy of the graph meanwhile
   if (e->opt.count_negatives && !e->neg_s.empty()) {
     const uint32_t nn = (uint32_t)e->neg_s.size();
     if ((rc = ensure_seed_cap(e, nn))) return rc;
     DevBuf<uint32_t> &rb = e->rec_neg;
     if ((rc = upload_records(e, e->neg_s, e->neg_d, rb))) return rc;
+    if (e->overlap_apply) {
+      // the update refills the staging buffer the records came from
+      HIPCHK(hipStreamSynchronize(e->stream));
+      HIPCHK(hipEventRecord(e->ev_pre, e->stream));
+    }
     if ((rc = run_round(e, rb.p, rb.p + nn, nn, 0, e->ev[6], e->ev[7], e->ev[8]))) return rc;
+    if (e->overlap_apply) {
+      HostDelta d;
+      host_delta(e, changes, k, d);
+      if ((rc = overlap_apply(e, d, true))) return rc;
+      applied = true;
+    }
     HIPCHK(hipStreamSynchronize(e->stream));
     if ((rc = check_round(e, &e->stats.seeds_neg))) return rc;
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e->ev[7], e->ev[8]));
     e->stats.ms_dfs_neg = ms;
   }
-  // 2-3. apply + relabel + filter
-  HIPCHK(hipEventRecord(e->ev[0], e->stream));
-  if ((rc = apply_and_refresh(e, changes, k))) return rc;
-  HIPCHK(hipEventRecord(e->ev[2], e->stream));
+  // 2-3. apply + relabel + filter (update, relabel, refilter and
+  // label-sorted copies; already done beside the negatives with overlap)
+  if (!applied) {
+    HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    if ((rc = apply_and_refresh(e, changes, k))) return rc;
+    HIPCHK(hipEventRecord(e->ev[2], e->stream));
+  }
   HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
-  e->stats.ms_apply = ms;  // update, relabel, refilter and label-sorted copies
+  if (!applied) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
+    e->stats.ms_apply = ms;
+  }
   e->stats.ms_filter = 0;
   e->stats.relabeled_vertices = e->h_actl.p->relabeled;
   e->stats.dirty_groups = e->h_actl.p->ndirty;
