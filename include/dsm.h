@@ -94,11 +94,12 @@ typedef struct {
   int graph_rounds;          /* 1: each DFS round (control reset, seeding, DFS, status
                                 copy) is one HIP graph launch, built once and updated
                                 per round; 0: separate launches */
-  int overlap_apply;         /* 1: pipelined batches (dsm_process_batches, the served
-                                stream) apply the next batch to a second copy of the
-                                graph while the round matches on the first, which
-                                doubles the graph's device memory; 0: apply after the
-                                round */
+  int overlap_apply;         /* 1: the update is applied to a second copy of the graph
+                                while a round matches on the first: the next batch's
+                                during pipelined rounds (dsm_process_batches, the served
+                                stream), the batch's own during its negatives round
+                                (dsm_apply_batch). Doubles the graph's device memory;
+                                0: apply after the round */
   uint32_t leaf_budget;      /* one level above the leaf: inline leaf-count work (list
                                 entries) of a 64-candidate window before a candidate
                                 descends so that its scan can be shared with idle waves
