@@ -585,3 +585,22 @@ def test_query_time_reports_and_resets():
         assert np.all(eng.query_time() == 0)
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("stats", [False, True])
+def test_cdsm_cli_counts(golden_dir, stats):
+    """The CDSM command line on the 200-batch small fixture prints, per batch,
+    the expected `res:` and `neg:` counts, pipelined (no --stats) or batch by
+    batch (--stats)."""
+    import subprocess
+    say("case: CDSM command line, stats=%s" % stats)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dsm_amd", "bin", "CDSM")
+    exp = json.load(open(os.path.join(golden_dir, "small_expected.json")))
+    cmd = [exe, "-q", os.path.join(golden_dir, "small_q.in"), "-d", os.path.join(golden_dir, "small_g.in"),
+           "--gpu", "0", "--neg"] + (["--stats"] if stats else [])
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = [int(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("res:")]
+    neg = [int(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("neg:")]
+    assert res == [b["pos"] for b in exp["batches"]]
+    assert neg == [b["neg"] for b in exp["batches"]]
