@@ -264,7 +264,7 @@ def run_served(w, rates, first_batch, engine_kw, seed=7, max_requests=160, offer
             span = max(i["t_done"] for i in ok) - min(i["t_submit"] for i in infos)
             recs = int(sum(i["num_changes"] for i in ok))
             # the sizes drawn, over the arrival span: the load actually offered
-            realized = float(sizes.sum()) / arrive[-1] if arrive[-1] > 0 else None
+            realized = float(sizes.sum()) / float(arrive[-1]) if arrive[-1] > 0 else None
             r = {"offered": rate, "offered_realized": round(realized, 1) if realized else None,
                  "requests": nreq, "rejected": nreq - len(ok), "records": recs,
                  "value": round(recs / span, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
