@@ -587,20 +587,21 @@ def test_query_time_reports_and_resets():
         eng.close()
 
 
-@pytest.mark.parametrize("stats", [False, True])
-def test_cdsm_cli_counts(golden_dir, stats):
+@pytest.mark.parametrize("stats,neg", [(False, True), (True, True), (False, False)])
+def test_cdsm_cli_counts(golden_dir, stats, neg):
     """The CDSM command line on the 200-batch small fixture prints, per batch,
-    the expected `res:` and `neg:` counts, pipelined (no --stats) or batch by
-    batch (--stats)."""
+    the expected `res:` counts (and `neg:` counts with --neg; without it the
+    engine matches no deletes), pipelined (no --stats) or batch by batch
+    (--stats)."""
     import subprocess
-    say("case: CDSM command line, stats=%s" % stats)
+    say("case: CDSM command line, stats=%s neg=%s" % (stats, neg))
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dsm_amd", "bin", "CDSM")
     exp = json.load(open(os.path.join(golden_dir, "small_expected.json")))
     cmd = [exe, "-q", os.path.join(golden_dir, "small_q.in"), "-d", os.path.join(golden_dir, "small_g.in"),
-           "--gpu", "0", "--neg"] + (["--stats"] if stats else [])
+           "--gpu", "0"] + (["--neg"] if neg else []) + (["--stats"] if stats else [])
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     res = [int(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("res:")]
-    neg = [int(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("neg:")]
+    negs = [int(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("neg:")]
     assert res == [b["pos"] for b in exp["batches"]]
-    assert neg == [b["neg"] for b in exp["batches"]]
+    assert negs == ([b["neg"] for b in exp["batches"]] if neg else [])
