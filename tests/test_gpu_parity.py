@@ -605,3 +605,36 @@ def test_cdsm_cli_counts(golden_dir, stats, neg):
     negs = [int(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("neg:")]
     assert res == [b["pos"] for b in exp["batches"]]
     assert negs == ([b["neg"] for b in exp["batches"]] if neg else [])
+
+
+def test_tuple_capacity_truncation():
+    """More tuples than tuple_capacity: the counts stay exact, dsm_get_tuples
+    reports the number found and DSM_ETRUNC, and every tuple kept is one of
+    the oracle's embeddings."""
+    import ctypes
+    say("case: tuple capacity truncation")
+    rng = np.random.default_rng(2020)
+    n, m, nl = 300, 3000, 2
+    labels, eu, ev = random_graph(n, m, nl, rng)
+    queries = [qtuple("triangle", [0, 1, 0])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    batch = conflict_free_batch(n, 300, present, rng, p_add=1.0)
+    g = O.Graph(n, labels, eu, ev)
+    want = O.run_stream(g, oracle_queries(queries), [batch], relabel=True, tuples=True)[0]
+    total = int(sum(want["pos"]) + sum(want["neg"]))
+    cap = 16
+    assert total > cap
+    eng = E.Engine((n, labels, eu, ev), queries, relabel=True, tuple_capacity=cap)
+    try:
+        pos, neg = eng.process_batch(*batch)
+        assert pos.tolist() == want["pos"] and neg.tolist() == want["neg"]
+        num = ctypes.c_uint64()
+        buf = np.zeros(cap * E.TUPLE_WORDS, np.uint32)
+        rc = E.lib().dsm_get_tuples(eng._h, buf.ctypes.data, cap, ctypes.byref(num))
+        assert rc == E.DSM_ETRUNC and num.value == total
+        real = set(map(tuple, want["pos_tuples"][0].tolist()))
+        for row in buf.reshape(cap, E.TUPLE_WORDS):
+            assert row[0] == 0 and row[1] == 1
+            assert tuple(row[2:5].tolist()) in real
+    finally:
+        eng.close()
