@@ -12,7 +12,7 @@ Legs (one JSON line on stdout, details on stderr):
   batched  one engine, all 64 queries matched per launch (the headline value)
   plain    the drop-in path as the reference runs it: one query per engine
            (one CDSM process per query), each engine doing the whole batch
-           pipeline; run over the first --plain-steps timed batches
+           pipeline; run over the same timed batches as the batched leg
   served   the same stream cut into requests of log-uniform size (1..16,384
            records) arriving as a Poisson process at swept edge rates,
            served by one dsm_server (continuous batching, admission by
@@ -296,20 +296,25 @@ def run_cpu_threads(w, batch_index, threads):
 
     def phase(ex, sel):
         # candidate sets per query, then (query, 1/8 of the records) tasks so
-        # that one heavy query does not serialise the phase
+        # that one heavy query does not serialise the phase; per-query sums
         cands = list(ex.map(g.candidates, oq))
         s_, d_ = src[sel], dst[sel]
         parts = np.array_split(np.arange(len(s_)), 8)
         tasks = [(qi, p) for qi in range(len(oq)) for p in parts if len(p)]
-        list(ex.map(lambda t: g.count_records(oq[t[0]], s_[t[1]], d_[t[1]], cands[t[0]]), tasks))
+        sums = list(ex.map(lambda t: int(g.count_records(oq[t[0]], s_[t[1]], d_[t[1]], cands[t[0]]).sum()),
+                           tasks))
+        tot = np.zeros(len(oq), np.uint64)
+        for (qi, _), v in zip(tasks, sums):
+            tot[qi] += np.uint64(v)
+        return tot
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        phase(ex, dels)
+        neg = phase(ex, dels)
         g.apply(add, src, dst, lab, True)
-        phase(ex, ~dels)
+        pos = phase(ex, ~dels)
     t = time.perf_counter() - t0
-    return {"value": len(src) / t, "seconds": t,
+    return {"value": len(src) / t, "seconds": t, "pos": pos, "neg": neg,
             "sample": "batch %d (%d records), all %d queries, %d threads over (query, record share) tasks"
                       % (batch_index, len(src), len(oq), threads)}
 
@@ -379,7 +384,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat1m_q64")
-    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=0, help="0: the same batches as the batched leg (--steps)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-one-by-one", action="store_true",
                     help="skip the leg that calls dsm_process_batch once per batch")
@@ -488,7 +493,7 @@ def main():
                            "waves_per_launch": int(agg["waves"] / nl)}}
 
     if not args.no_plain:
-        ps = min(args.plain_steps, args.steps)
+        ps = min(args.plain_steps, args.steps) if args.plain_steps else args.steps
         pl = run_plain(w, args.warmup, ps, engine_kw)
         # the plain path must produce the same counts
         for b in range(args.warmup + ps):
@@ -523,6 +528,10 @@ def main():
         # host), and on one core with a time budget
         threads = max(1, min(16, os.cpu_count() or 1))
         cpu_n = run_cpu_threads(w, args.warmup, threads)
+        # the oracle's counts of that batch are the GPU's (first timed batch)
+        gp, gn = bat["per_batch"][args.warmup]
+        oracle_ok = bool(np.array_equal(cpu_n["pos"], gp) and np.array_equal(cpu_n["neg"], gn))
+        assert oracle_ok, "GPU counts differ from the oracle's at batch %d" % args.warmup
         cpu_1 = run_cpu(w, args.warmup, args.cpu_budget)
         log("cpu: %d threads %.1f updates/s; 1 thread %.1f updates/s" % (threads, cpu_n["value"], cpu_1["value"]))
         model = ""
@@ -533,6 +542,7 @@ def main():
             pass
         out["cpu_baseline"] = {"value": round(cpu_n["value"], 2), "unit": "edge updates/s", "cores": threads,
                                "kind": "port", "sample": cpu_n["sample"],
+                               "counts_equal_gpu": oracle_ok,
                                "one_core": {"value": round(cpu_1["value"], 2), "sample": cpu_1["sample"]},
                                "host": {"nproc": os.cpu_count(), "cpu_model": model}}
     print(json.dumps(out), flush=True)

@@ -36,9 +36,9 @@ def build_engine():
     objs = []
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    for src in ("dsm_kernels.hip", "dsm_engine.cpp", "dsm_file.cpp"):
+    for src in ("dsm_kernels.hip", "dsm_cluster.hip", "dsm_engine.cpp", "dsm_cluster_plan.cpp", "dsm_file.cpp"):
         obj = os.path.join(objdir, src + ".o")
-        lang = [] if src.endswith(".hip") else ["-x", "hip"] if src == "dsm_engine.cpp" else []
+        lang = [] if src.endswith(".hip") else ["-x", "hip"] if src in ("dsm_engine.cpp", "dsm_cluster_plan.cpp") else []
         _run([HIPCC] + HIP_FLAGS + lang + ["-I", os.path.join(ROOT, "include"), "-c",
                                           os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
@@ -54,7 +54,7 @@ def build_engine():
 
 def build_gen():
     lib = os.path.join(HERE, "libdsm_gen.so")
-    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-o", lib, os.path.join(CSRC, "gen.cpp")])
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", lib, os.path.join(CSRC, "gen.cpp")])
     return lib
 
 

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -129,6 +130,127 @@ int64_t gen_edges_rmat(uint32_t n, uint64_t m, double a, double b, double c, uin
   for (uint32_t i = 0; i < n; i++) perm[i] = i;
   for (uint32_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[r.below(i + 1)]);
   for (uint64_t i = 0; i < m; i++) { eu[i] = perm[eu[i]]; ev[i] = perm[ev[i]]; }
+  return (int64_t)k;
+}
+
+// R-MAT as above, generated in parallel with the same result for any thread
+// count. Candidate edges come in chunks of 2^20 draws, chunk c from its own
+// generator seeded by (seed, c); the sequence is chunk 0, 1, 2, ... in order.
+// Rejected draws (ids >= n, self loops) are dropped, and of the rest the first
+// m distinct undirected edges in sequence order are kept. Duplicates are found
+// by key-sharded hash sets: every thread scans a round's candidates in order
+// and inserts those of its own shards, so "first in sequence order" holds
+// without locks. Ids are then relabelled by a seeded permutation.
+int64_t gen_edges_rmat_chunked(uint32_t n, uint64_t m, double a, double b, double c, uint64_t seed,
+                               uint32_t threads, uint32_t *eu, uint32_t *ev) {
+  if (n < 2) return -1;
+  int scale = 0;
+  while ((1ull << scale) < n) scale++;
+  if (threads == 0) threads = 1;
+  if (threads > 64) threads = 64;
+  const uint64_t kChunk = 1ull << 20;
+  const uint32_t kShards = 256;
+  const double ab = a + b, abc = a + b + c;
+  // shard tables: open addressing, capacity ~2.2x the expected keys per shard
+  const uint64_t per = (uint64_t)((double)m / kShards * 2.2) + 1024;
+  std::vector<std::vector<uint64_t>> tab(kShards, std::vector<uint64_t>());
+  std::vector<uint64_t> used(kShards, 0);
+  auto shard_of = [](uint64_t h) { return (uint32_t)(h >> 56); };
+  auto slot_of = [per](uint64_t h) { return (uint64_t)(((__uint128_t)(h << 8) * per) >> 64); };
+  std::vector<uint64_t> cand;
+  std::vector<uint8_t> first;
+  uint64_t k = 0, next_chunk = 0;
+  int fail = 0;
+  while (k < m) {
+    uint64_t need = m - k;
+    uint64_t nch = (uint64_t)((double)need * 1.3 / (double)kChunk) + 1;
+    if (next_chunk + nch > (m * 64) / kChunk + 64) return -2;
+    cand.assign(nch * kChunk, ~0ull);
+    first.assign(nch * kChunk, 0);
+    // 1. draws, chunk by chunk (chunks split over the threads)
+    {
+      std::vector<std::thread> th;
+      for (uint32_t t = 0; t < threads; t++)
+        th.emplace_back([&, t]() {
+          for (uint64_t ci = t; ci < nch; ci += threads) {
+            Rng r(seed * 0x9E3779B97F4A7C15ull + (next_chunk + ci) * 0xD1B54A32D192ED03ull + 0x5851F42D4C957F2Dull);
+            uint64_t *out = cand.data() + ci * kChunk;
+            for (uint64_t i = 0; i < kChunk; i++) {
+              uint32_t u = 0, v = 0;
+              for (int l = 0; l < scale; l++) {
+                double x = r.uniform();
+                uint32_t bu = 0, bv = 0;
+                if (x < a) {
+                } else if (x < ab) {
+                  bv = 1;
+                } else if (x < abc) {
+                  bu = 1;
+                } else {
+                  bu = 1; bv = 1;
+                }
+                u = (u << 1) | bu;
+                v = (v << 1) | bv;
+              }
+              out[i] = (u >= n || v >= n || u == v) ? ~0ull : ekey(u, v);
+            }
+          }
+        });
+      for (auto &x : th) x.join();
+    }
+    next_chunk += nch;
+    // 2. first occurrences, thread t owning shards t, t+threads, ...
+    {
+      std::vector<std::thread> th;
+      std::vector<int> bad(threads, 0);
+      for (uint32_t t = 0; t < threads; t++)
+        th.emplace_back([&, t]() {
+          for (uint32_t s = t; s < kShards; s += threads)
+            if (tab[s].empty()) tab[s].assign(per, ~0ull);
+          for (uint64_t i = 0; i < cand.size(); i++) {
+            uint64_t key = cand[i];
+            if (key == ~0ull) continue;
+            uint64_t h = EdgeSet::h(key);
+            uint32_t s = shard_of(h);
+            if (s % threads != t) continue;
+            std::vector<uint64_t> &T = tab[s];
+            uint64_t j = slot_of(h);
+            while (T[j] != ~0ull && T[j] != key) j = (j + 1 == per) ? 0 : j + 1;
+            if (T[j] == key) continue;
+            if (++used[s] * 10 > per * 9) { bad[t] = 1; return; }
+            T[j] = key;
+            first[i] = 1;
+          }
+        });
+      for (auto &x : th) x.join();
+      for (int x : bad) fail |= x;
+    }
+    if (fail) return -3;
+    // 3. the first occurrences in sequence order, up to m edges
+    for (uint64_t i = 0; i < cand.size() && k < m; i++)
+      if (first[i]) {
+        eu[k] = (uint32_t)(cand[i] >> 32);
+        ev[k] = (uint32_t)cand[i];
+        k++;
+      }
+    if (g_progress) {
+      fprintf(stderr, "R-MAT (chunked): %llu of %llu edges after %llu chunks\n", (unsigned long long)k,
+              (unsigned long long)m, (unsigned long long)next_chunk);
+      fflush(stderr);
+    }
+  }
+  Rng r(seed ^ 0xA0761D6478BD642Full);
+  std::vector<uint32_t> perm(n);
+  for (uint32_t i = 0; i < n; i++) perm[i] = i;
+  for (uint32_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[r.below(i + 1)]);
+  {
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < threads; t++)
+      th.emplace_back([&, t]() {
+        uint64_t lo = m * t / threads, hi = m * (t + 1) / threads;
+        for (uint64_t i = lo; i < hi; i++) { eu[i] = perm[eu[i]]; ev[i] = perm[ev[i]]; }
+      });
+    for (auto &x : th) x.join();
+  }
   return (int64_t)k;
 }
 

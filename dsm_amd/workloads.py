@@ -51,6 +51,9 @@ def _lib():
     L.gen_edges_rmat.restype = C.c_int64
     L.gen_edges_rmat.argtypes = [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,
                                  C.c_uint64, u32p, u32p]
+    L.gen_edges_rmat_chunked.restype = C.c_int64
+    L.gen_edges_rmat_chunked.argtypes = [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                         C.c_uint64, C.c_uint32, u32p, u32p]
     L.gen_labels.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_uint64, u32p]
     L.gen_set_progress.argtypes = [C.c_int]
     # graphs of more than 2^25 edges report generation progress on stderr
@@ -251,13 +254,17 @@ def small(seed=1):
 _RMAT_CACHE = {}
 
 
-def _rmat_graph(n, m, nl, s, seed):
-    key = (n, m, nl, s, seed)
+def _rmat_graph(n, m, nl, s, seed, chunked=False):
+    key = (n, m, nl, s, seed, chunked)
     if key not in _RMAT_CACHE:
         L = _lib()
         eu = np.zeros(m, np.uint32)
         ev = np.zeros(m, np.uint32)
-        rc = L.gen_edges_rmat(n, m, 0.57, 0.19, 0.19, seed, eu, ev)
+        if chunked:
+            threads = max(1, min(16, os.cpu_count() or 1))
+            rc = L.gen_edges_rmat_chunked(n, m, 0.57, 0.19, 0.19, seed, threads, eu, ev)
+        else:
+            rc = L.gen_edges_rmat(n, m, 0.57, 0.19, 0.19, seed, eu, ev)
         if rc != m:
             raise RuntimeError("R-MAT generation failed: %d" % rc)
         labels = np.zeros(n, np.uint32)
@@ -347,11 +354,13 @@ def load_queries(name):
 
 
 def rmat(n=1000000, m=10000000, nl=16, zipf=1.0, batch=4096, seed=2, frac=0.10, name="rmat1m",
-         queries=None):
+         queries=None, chunked=False):
     """R-MAT graph with a held-out insert stream and random deletes; the
     standing queries are the committed set `queries/<name>.json` (drawn by
-    draw_queries, see queries/draw.py) unless a list is given."""
-    eu, ev, labels = _rmat_graph(n, m, nl, zipf, seed)
+    draw_queries, see queries/draw.py) unless a list is given. chunked: the
+    parallel generator (gen_edges_rmat_chunked; a different, equally seeded
+    edge sequence)."""
+    eu, ev, labels = _rmat_graph(n, m, nl, zipf, seed, chunked)
     n_ins = int(m * frac)
     keep, recs = _stream(m, n_ins, n_ins, batch, seed + 20, interleave=True)
     w = _assemble(name, n, labels, eu, ev, keep, recs, batch, [], seed + 40)
@@ -388,4 +397,4 @@ def rmat16m(**kw):
     batches of 32,768, with the committed dense queries (4-clique, 5-cycle with
     two chords, 7-vertex tree)."""
     kw.setdefault("name", "rmat16m")
-    return rmat(n=16_000_000, m=256_000_000, nl=32, zipf=0.8, batch=32768, seed=5, **kw)
+    return rmat(n=16_000_000, m=256_000_000, nl=32, zipf=0.8, batch=32768, seed=5, chunked=True, **kw)

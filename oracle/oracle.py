@@ -54,6 +54,10 @@ def lib():
     L.og_brute_record.argtypes = [vp, vp, C.c_uint32, C.c_uint32]
     L.og_brute_total.restype = C.c_uint64
     L.og_brute_total.argtypes = [vp, vp, vp, C.c_uint64]
+    L.oq_cluster_plan.restype = C.c_uint32
+    L.oq_cluster_plan.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, u32p, C.c_uint32]
+    L.og_cluster_filter.restype = C.c_int
+    L.og_cluster_filter.argtypes = [vp, C.c_uint32, C.c_uint32, u32p, u32p, u8p, u8p]
     L.of_read.restype = vp
     L.of_read.argtypes = [C.c_char_p, C.c_int]
     L.of_free.argtypes = [vp]
@@ -130,6 +134,21 @@ class Graph:
         L.og_tuples_records(self._h, q._h, c, src.shape[0], src, dst, buf.ctypes.data, n)
         return buf[: n * q.n].reshape(n, q.n)
 
+    def cluster_candidates(self, q, ldf=None):
+        """Cluster-encoding refinement of q's candidates (clusterFilter,
+        filter.cu:1525-1654) as q.n rows of bytes; the plan follows q's edge
+        lines in the order given."""
+        if ldf is None:
+            ldf = self.candidates(q)
+        eu = _u32([a for a, _ in q.edges])
+        ev = _u32([b for _, b in q.edges])
+        out = np.zeros(q.n * self.n, np.uint8)
+        rc = lib().og_cluster_filter(self._h, q.n, eu.shape[0], eu, ev,
+                                     np.ascontiguousarray(ldf.reshape(-1), np.uint8), out)
+        if rc != 0:
+            raise ValueError("cluster plan exceeds a limit of the reference")
+        return out.reshape(q.n, self.n)
+
     def brute_record(self, q, s, d):
         return int(lib().og_brute_record(self._h, q._h, int(s), int(d)))
 
@@ -159,6 +178,43 @@ class Query:
         if getattr(self, "_h", None):
             lib().oq_free(self._h)
             self._h = None
+
+
+OP_NAMES = {0: "encode", 1: "merge", 2: "combine"}
+
+
+def cluster_plan_words(n, edges):
+    """The oracle's plan in the flat word layout (oq_cluster_plan)."""
+    eu = _u32([a for a, _ in edges])
+    ev = _u32([b for _, b in edges])
+    buf = np.zeros(1 << 16, np.uint32)
+    w = lib().oq_cluster_plan(n, eu.shape[0], eu, ev, buf, buf.shape[0])
+    return buf[:w].copy() if w else None
+
+
+def cluster_plan(n, edges):
+    """The cluster plan of a query (decycle.cpp:6-135, filter.cu:23-392,
+    encode()'s pass order filter.cu:936-1177, enc_pos_of_u filter.cu:1589-1609)
+    as a dict, for edge lines in the order given; None past a reference limit."""
+    eu = _u32([a for a, _ in edges])
+    ev = _u32([b for _, b in edges])
+    buf = np.zeros(1 << 16, np.uint32)
+    w = lib().oq_cluster_plan(n, eu.shape[0], eu, ev, buf, buf.shape[0])
+    if w == 0:
+        return None
+    it = iter(buf[:w].tolist())
+    nx = lambda: next(it)  # noqa: E731
+    out = {"n": nx()}
+    out["cover"] = [nx() for _ in range(nx())]
+    clusters = []
+    for _ in range(nx()):
+        valid = nx()
+        clusters.append({"valid": bool(valid), "members": [nx() for _ in range(nx())]})
+    out["clusters"] = clusters
+    out["final"] = nx()
+    out["enc_pos"] = [nx() for _ in range(out["n"])]
+    out["ops"] = [(OP_NAMES[nx()], nx()) for _ in range(nx())]
+    return out
 
 
 class DsmFile:
@@ -191,6 +247,38 @@ class DsmFile:
     def batch(self, b):
         s, e = int(self.bstart[b]), int(self.bstart[b + 1])
         return self.add[s:e], self.src[s:e], self.dst[s:e], self.clab[s:e]
+
+
+def run_stream_threaded(graph, queries, batches, relabel=True, threads=None, parts=8):
+    """run_stream's counts (no tuples) with the counting spread over host
+    threads: per phase, the candidate sets of every query, then (query, 1/parts
+    of the records) tasks. A count is a sum over records (main.cu:268-329), so
+    the split does not change it. The oracle's C calls release the GIL."""
+    from concurrent.futures import ThreadPoolExecutor
+    if threads is None:
+        threads = max(1, min(16, os.cpu_count() or 1))
+    out = []
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        def phase(s_, d_):
+            cands = list(ex.map(graph.candidates, queries))
+            chunks = [c for c in np.array_split(np.arange(len(s_)), parts) if len(c)]
+            tasks = [(qi, c) for qi in range(len(queries)) for c in chunks]
+            sums = list(ex.map(lambda t: int(graph.count_records(queries[t[0]], s_[t[1]], d_[t[1]],
+                                                                 cands[t[0]]).sum()), tasks))
+            tot = [0] * len(queries)
+            for (qi, _), v in zip(tasks, sums):
+                tot[qi] += v
+            return tot
+
+        for add, src, dst, lab in batches:
+            add = np.asarray(add, np.uint8)
+            src, dst = _u32(src), _u32(dst)
+            dels = add == 0
+            res = {"neg": phase(src[dels], dst[dels])}
+            graph.apply(add, src, dst, lab, relabel)
+            res["pos"] = phase(src[~dels], dst[~dels])
+            out.append(res)
+    return out
 
 
 def run_stream(graph, queries, batches, relabel=True, tuples=False):
