@@ -36,7 +36,8 @@ def build_engine():
     objs = []
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    for src in ("dsm_kernels.hip", "dsm_cluster.hip", "dsm_engine.cpp", "dsm_cluster_plan.cpp", "dsm_file.cpp"):
+    for src in ("dsm_kernels.hip", "dsm_cluster.hip", "dsm_intersect.hip", "dsm_engine.cpp", "dsm_cluster_plan.cpp",
+                "dsm_file.cpp"):
         obj = os.path.join(objdir, src + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"] if src in ("dsm_engine.cpp", "dsm_cluster_plan.cpp") else []
         _run([HIPCC] + HIP_FLAGS + lang + ["-I", os.path.join(ROOT, "include"), "-c",

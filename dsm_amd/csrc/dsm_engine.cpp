@@ -28,6 +28,21 @@ namespace {
 
 thread_local std::string g_create_error;
 
+}  // namespace
+
+namespace dsm {
+void set_global_error(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_create_error = buf;
+}
+}  // namespace dsm
+
+namespace {
+
 struct QueryHost {
   uint32_t n = 0;
   uint32_t lab[kMaxQ] = {};
@@ -138,6 +153,7 @@ struct dsm_engine {
   uint32_t dfs_blocks = 0, dfs_wpb = 4, max_qn = 2;
   uint32_t donate_min = 256, lane_work = 16, donate_every = 8, donate_min_n2 = 64, leaf_budget = 2048;
   bool leaf_sets = true;
+  bool bit_probe_last = false;
   bool graph_rounds = true;
   PinnedBuf<uint32_t> pipe_h_rec;  // records of a pipelined round (dsm_process_batches)
   DevBuf<uint32_t> pipe_d_rec;
@@ -175,8 +191,25 @@ struct dsm_engine {
   bool batch_pending = false;
   hipEvent_t ev[12] = {};
   dsm_stats stats{};
+  // cluster-encoding refinement (opt.cluster_filter): every query's position
+  // rows follow the signature rows in bm; the DFS and the seeding read the
+  // final positions
+  bool cluster = false;
+  uint32_t nsig_rows = 0, npos_rows = 0;
+  std::vector<std::vector<uint32_t>> qref;  // per query, per vertex: the bitmap row the DFS uses
+  std::vector<uint32_t> cstep_off;          // ops of pass step s: [cstep_off[s], cstep_off[s + 1])
+  uint32_t cmax_step_ops = 0;
+  DevBuf<ClusterOp> d_cops;
+  DevBuf<uint32_t> d_crow, d_alive_src, cctl;
+  uint32_t nalive_src = 0, nalive_rows = 0;
+  DevBuf<uint64_t> calive;
+  DevBuf<ClusterHeavy> cheavy;
+  hipEvent_t ev_ref[2] = {};
+  bool refine_timed = false;
 
   ~dsm_engine() {
+    for (auto &x : ev_ref)
+      if (x) (void)hipEventDestroy(x);
     if (rg_exec) (void)hipGraphExecDestroy(rg_exec);
     if (rg_graph) (void)hipGraphDestroy(rg_graph);
     for (auto &e : ev)
@@ -258,6 +291,123 @@ struct dsm_engine {
 namespace {
 
 // ------------------------------------------------------------ query set-up
+
+// The refinement's passes for every query (dsm_cluster_plan.cpp), as device
+// ops grouped by pass step: step s holds pass s of every query that has one.
+// Rows: position p of query q is bitmap row nsig + qbase[q] + p; the alive row
+// of query vertex u of q is abase[q] + u.
+int build_cluster_ops(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq) {
+  std::vector<ClusterPlan> plans(nq);
+  std::vector<uint32_t> qbase(nq), abase(nq);
+  uint32_t npos = 0, nalive = 0, nsteps = 0;
+  for (uint32_t q = 0; q < nq; q++) {
+    if (build_cluster_plan(qs[q].num_vertices, qs[q].num_edges, qs[q].edge_src, qs[q].edge_dst, plans[q]))
+      return e->fail_msg(DSM_ELIMIT, "query %u: its cluster plan exceeds the reference's limits (%u clusters, "
+                         "%u small clusters per combine)", q, kMaxClusters, kMaxSmallClusters);
+    qbase[q] = npos;
+    abase[q] = nalive;
+    npos += plans[q].positions;
+    nalive += e->queries[q].n;
+    nsteps = std::max<uint32_t>(nsteps, (uint32_t)plans[q].passes.size());
+  }
+  if ((uint64_t)e->nsig_rows + npos > 65535)
+    return e->fail_msg(DSM_ELIMIT, "cluster positions of the queries exceed 65,535 bitmap rows");
+  const uint32_t nsig = e->nsig_rows;
+  auto row = [&](uint32_t q, uint32_t p) { return nsig + qbase[q] + p; };
+  std::vector<std::vector<ClusterOp>> steps(nsteps);
+  std::vector<uint32_t> crow;
+  std::vector<bool> core_alive(nalive, false);
+  for (uint32_t q = 0; q < nq; q++) {
+    const ClusterPlan &P = plans[q];
+    const QueryHost &Q = e->queries[q];
+    for (size_t s = 0; s < P.passes.size(); s++) {
+      const auto &pass = P.passes[s];
+      ClusterOp op{};
+      const ClusterPlan::Cluster &c =
+          P.clusters[pass.kind == ClusterPlan::kCombine ? P.combines[pass.arg].big : pass.arg];
+      const uint32_t core = c.members[0];
+      op.alive = abase[q] + core;
+      core_alive[op.alive] = true;
+      op.core = row(q, c.pos);
+      op.ntarget = (uint32_t)c.members.size() - 1;
+      op.check_off = (uint32_t)crow.size();
+      for (uint32_t t = 0; t < op.ntarget; t++) op.tlab[t] = (uint8_t)Q.lab[c.members[t + 1]];
+      if (pass.kind == ClusterPlan::kEncode) {
+        // encodeKernel: a neighbour of a core candidate is a target's candidate
+        op.kind = kOpEncode;
+        for (uint32_t t = 0; t < op.ntarget; t++) {
+          op.toff[t] = (uint16_t)crow.size();
+          op.tcnt[t] = 1;
+          crow.push_back(e->qsig[q][c.members[t + 1]]);
+        }
+      } else if (pass.kind == ClusterPlan::kMerge) {
+        // mergeKernel: the core in both merged clusters; the targets are the
+        // merged clusters' cores
+        op.kind = kOpMerge;
+        crow.push_back(row(q, c.left_pos));
+        crow.push_back(row(q, c.right_pos));
+        op.ncheck = 2;
+        const uint32_t src[2] = {P.clusters[c.left].pos, P.clusters[c.right].pos};
+        for (uint32_t t = 0; t < op.ntarget; t++) {
+          op.toff[t] = (uint16_t)crow.size();
+          op.tcnt[t] = 1;
+          crow.push_back(row(q, src[t]));
+        }
+      } else {
+        // combineMultipleClustersKernel: the core in every small cluster; a
+        // target in every small cluster that holds it
+        const ClusterPlan::Combine &cb = P.combines[pass.arg];
+        op.kind = cb.keep ? kOpCombineKeep : kOpCombineNew;
+        for (uint32_t sm : cb.small) crow.push_back(row(q, P.clusters[sm].pos));
+        op.ncheck = (uint32_t)cb.small.size();
+        for (uint32_t t = 0; t < op.ntarget; t++) {
+          op.toff[t] = (uint16_t)crow.size();
+          const uint32_t tu = c.members[t + 1];
+          for (uint32_t sm : cb.small) {
+            const auto &mm = P.clusters[sm].members;
+            for (size_t x = 1; x < mm.size(); x++)
+              if (mm[x] == tu) {
+                crow.push_back(row(q, P.clusters[sm].pos + (uint32_t)x));
+                break;
+              }
+          }
+          op.tcnt[t] = (uint16_t)(crow.size() - op.toff[t]);
+        }
+      }
+      if (crow.size() > 65535) return e->fail_msg(DSM_ELIMIT, "cluster plans too large");
+      steps[s].push_back(op);
+    }
+    for (uint32_t u = 0; u < Q.n; u++) e->qref[q][u] = row(q, P.final_pos[u]);
+  }
+  std::vector<ClusterOp> ops;
+  e->cstep_off.assign(1, 0);
+  e->cmax_step_ops = 0;
+  for (auto &st : steps) {
+    ops.insert(ops.end(), st.begin(), st.end());
+    e->cstep_off.push_back((uint32_t)ops.size());
+    e->cmax_step_ops = std::max<uint32_t>(e->cmax_step_ops, (uint32_t)st.size());
+  }
+  // alive rows: every core's candidate list starts as its LDF/NLF row
+  std::vector<uint32_t> asrc;
+  for (uint32_t q = 0; q < nq; q++)
+    for (uint32_t u = 0; u < e->queries[q].n; u++)
+      if (core_alive[abase[q] + u]) {
+        asrc.push_back(abase[q] + u);
+        asrc.push_back(e->qsig[q][u]);
+      }
+  e->npos_rows = npos;
+  e->nalive_rows = nalive;
+  e->nalive_src = (uint32_t)asrc.size() / 2;
+  HIPCHK(e->d_cops.alloc(ops.size()));
+  HIPCHK(hipMemcpy(e->d_cops.p, ops.data(), ops.size() * sizeof(ClusterOp), hipMemcpyHostToDevice));
+  HIPCHK(e->d_crow.alloc(crow.size()));
+  if (!crow.empty()) HIPCHK(hipMemcpy(e->d_crow.p, crow.data(), crow.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(e->d_alive_src.alloc(asrc.size()));
+  if (!asrc.empty()) HIPCHK(hipMemcpy(e->d_alive_src.p, asrc.data(), asrc.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(e->cctl.alloc(2 * (size_t)std::max<uint32_t>(nsteps, 1)));
+  for (auto &x : e->ev_ref) HIPCHK(hipEventCreate(&x));
+  return DSM_OK;
+}
 
 // label_freq[l]: data vertices of label l (l < 255), used only to order the
 // matching: it never changes a count.
@@ -346,6 +496,14 @@ int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq, const st
     if (e->sig_hi[l] == 0) e->sig_lo[l] = i;
     e->sig_hi[l] = i + 1;
   }
+  // cluster-encoding refinement: position rows after the signature rows
+  e->nsig_rows = (uint32_t)e->sigs.size();
+  e->qref.assign(nq, std::vector<uint32_t>());
+  for (uint32_t q = 0; q < nq; q++)
+    for (uint32_t u = 0; u < e->queries[q].n; u++) e->qref[q].push_back(e->qsig[q][u]);
+  if (e->cluster) {
+    if (int rc = build_cluster_ops(e, qs, nq)) return rc;
+  }
   // plans: every query edge in both orientations
   struct PK {
     uint32_t key;
@@ -385,7 +543,7 @@ int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq, const st
         }
         for (uint32_t k = 0; k < Q.n; k++) {
           P.qv[k] = (uint8_t)order[k];
-          P.sig[k] = e->qsig[q][order[k]];
+          P.sig[k] = (uint16_t)e->qref[q][order[k]];
           uint16_t back = 0, inj = 0;
           for (uint32_t j = 0; j < k; j++) {
             if (Q.adj[order[k]] >> order[j] & 1) back |= (uint16_t)(1u << j);
@@ -504,8 +662,10 @@ int upload_graph(dsm_engine *e, const dsm_graph_desc *g) {
 
 int upload_tables(dsm_engine *e) {
   e->words = (e->n + 63) / 64;
-  HIPCHK(e->bm.alloc((size_t)e->sigs.size() * e->words));
-  HIPCHK(hipMemset(e->bm.p, 0, (size_t)e->sigs.size() * e->words * 8));
+  const size_t rows = (size_t)e->nsig_rows + e->npos_rows;
+  HIPCHK(e->bm.alloc(rows * e->words));
+  HIPCHK(hipMemset(e->bm.p, 0, rows * e->words * 8));
+  if (e->cluster) HIPCHK(e->calive.alloc((size_t)e->nalive_rows * e->words));
   HIPCHK(e->d_sigs.alloc(e->sigs.size()));
   HIPCHK(hipMemcpy(e->d_sigs.p, e->sigs.data(), e->sigs.size() * sizeof(SigReq), hipMemcpyHostToDevice));
   HIPCHK(e->d_sig_lo.alloc(256));
@@ -607,6 +767,7 @@ DfsArgs dfs_args(dsm_engine *e) {
   da.donate_min = e->donate_min;
   da.lane_work = e->lane_work;
   da.leaf_budget = e->leaf_budget;
+  da.bit_probe_last = e->bit_probe_last ? 1u : 0u;
   da.donate_every = e->donate_every;
   da.donate_min_n2 = e->donate_min_n2;
   da.leaf_segments = e->leaf_segments ? 1u : 0u;
@@ -994,6 +1155,51 @@ int upload_records(dsm_engine *e, const std::vector<uint32_t> &s, const std::vec
 }
 
 
+// The cluster-encoding refinement of every query on the current graph,
+// enqueued on the engine's stream after the LDF/NLF rows are current: the
+// position rows are cleared, each core's alive row loaded, then pass step s of
+// every query runs as one k_cluster_pass launch plus one k_cluster_heavy
+// launch for the chunks of candidates with long lists (filter.cu:936-1177 runs
+// one kernel and one device synchronisation per cluster of one query).
+int cluster_refine(dsm_engine *e) {
+  if (!e->cluster) return DSM_OK;
+  const uint32_t nsteps = (uint32_t)e->cstep_off.size() - 1;
+  // heavy chunks of one pass: at most 2 x (list entries) / kClusterChunk
+  const unsigned long long entries = std::max<unsigned long long>(e->h_actl.p->pool_used, 1);
+  const size_t need = (size_t)e->cmax_step_ops * (2 * entries / kClusterChunk + 64);
+  if (need > e->cheavy.n) HIPCHK(e->cheavy.alloc(need));
+  HIPCHK(hipEventRecord(e->ev_ref[0], e->stream));
+  HIPCHK(launch_cluster_init(e->bm.p, e->words, e->nsig_rows, e->npos_rows, e->calive.p, e->d_alive_src.p,
+                             e->nalive_src, e->cctl.p, 2 * nsteps, e->stream));
+  ClusterArgs a{};
+  a.g = e->dev_graph();
+  a.bm = e->bm.p;
+  a.words = e->words;
+  a.alive = e->calive.p;
+  a.crow = e->d_crow.p;
+  a.heavy = e->cheavy.p;
+  a.heavy_cap = (uint32_t)std::min<size_t>(e->cheavy.n, 0xFFFFFFF0u);
+  for (uint32_t st = 0; st < nsteps; st++) {
+    a.ops = e->d_cops.p + e->cstep_off[st];
+    a.nops = e->cstep_off[st + 1] - e->cstep_off[st];
+    a.ctl = e->cctl.p + 2 * st;
+    HIPCHK(launch_cluster_pass(a, (uint32_t)e->num_cu * 8, e->stream));
+    HIPCHK(launch_cluster_heavy(a, (uint32_t)e->num_cu * 4, e->stream));
+  }
+  HIPCHK(hipEventRecord(e->ev_ref[1], e->stream));
+  e->refine_timed = true;
+  return DSM_OK;
+}
+
+// Adds the device time of the last refinement to the batch statistics (the
+// refinement's stream must have been synchronised).
+void take_refine_time(dsm_engine *e) {
+  if (!e->refine_timed) return;
+  e->refine_timed = false;
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, e->ev_ref[0], e->ev_ref[1]) == hipSuccess) e->stats.ms_refine += ms;
+}
+
 // One batch's update and candidate refresh from its host delta, enqueued on
 // the engine's stream: reset the per-batch apply counters, apply (device
 // merge/commit, which waits for the commit's status), relabel, rebuild the
@@ -1014,7 +1220,7 @@ int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d) {
                          e->stream));
   }
   HIPCHK(launch_filter_dirty(e->dev_graph(), e->tables(), e->dirty(), e->stream));
-  return DSM_OK;
+  return cluster_refine(e);
 }
 
 int apply_and_refresh(dsm_engine *e, const dsm_change *changes, uint32_t k) {
@@ -1058,6 +1264,7 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   if (e->wall_khz <= 0) e->wall_khz = 100000;
   if (e->opt.watchdog_ms) e->watchdog_ms = e->opt.watchdog_ms;
   for (auto &x : e->ev) HIPCHK(hipEventCreate(&x));
+  e->cluster = e->opt.cluster_filter != 0;
   std::vector<uint64_t> label_freq(256, 0);
   for (uint32_t v = 0; graph->labels && v < graph->num_vertices; v++)
     if (graph->labels[v] < 255) label_freq[graph->labels[v]]++;
@@ -1091,12 +1298,15 @@ int create_impl(dsm_engine *e, const dsm_graph_desc *graph, const dsm_query_desc
   e->donate_every = e->opt.donate_every ? e->opt.donate_every : 8;
   e->donate_min_n2 = e->opt.donate_min_n2 ? e->opt.donate_min_n2 : 64;
   e->leaf_sets = e->opt.leaf_sets != 0;
+  e->bit_probe_last = e->opt.bit_probe_last != 0;
   e->graph_rounds = e->opt.graph_rounds != 0;
   e->leaf_segments = e->opt.leaf_segments != 0;
   e->poll_max = e->opt.poll_max ? std::min<uint32_t>(e->opt.poll_max, 16u) : 16u;
   HIPCHK(launch_filter_full(e->dev_graph(), e->tables(), e->stream));
   HIPCHK(launch_lbuild(e->dev_graph(), e->d_lmap.p, nullptr, nullptr, e->huge_sort(), e->stream));
+  if ((rc = cluster_refine(e))) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
+  e->refine_timed = false;
   return DSM_OK;
 }
 
@@ -1162,7 +1372,8 @@ int copy_graph(dsm_engine *e, const dsm_engine::GraphCopy &c) {
   HIPCHK(copy_buf(e->cap, c.cap, c.cap.n, e->stream));
   HIPCHK(copy_buf(e->loff, c.loff, c.loff.n, e->stream));
   HIPCHK(copy_buf(e->lab, c.lab, c.lab.n, e->stream));
-  HIPCHK(copy_buf(e->bm, c.bm, c.bm.n, e->stream));
+  // the signature rows; the cluster position rows are recomputed by the update
+  HIPCHK(copy_buf(e->bm, c.bm, (size_t)e->nsig_rows * e->words, e->stream));
   HIPCHK(copy_buf(e->actl, c.actl, 1, e->stream));
   e->pool_cap = c.pool_cap;
   return DSM_OK;
@@ -1190,7 +1401,9 @@ int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats, bool keep = tru
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
       e->stats.ms_apply += ms;
+      take_refine_time(e);
     }
+    e->refine_timed = false;
     return DSM_OK;
   };
   rc = run();
@@ -1329,6 +1542,7 @@ int dsm_apply_batch(dsm_engine *e, const dsm_change *changes, uint32_t k) {
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
     e->stats.ms_apply = ms;
+    take_refine_time(e);
   }
   e->stats.ms_filter = 0;
   e->stats.relabeled_vertices = e->h_actl.p->relabeled;
@@ -1434,6 +1648,49 @@ int dsm_get_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words) {
   return DSM_OK;
 }
 
+uint32_t dsm_cluster_plan(uint32_t num_vertices, uint32_t num_edges, const uint32_t *edge_src,
+                          const uint32_t *edge_dst, uint32_t *out, uint32_t cap) {
+  if (!edge_src || !edge_dst || num_vertices > (uint32_t)kMaxQ) return 0;
+  for (uint32_t i = 0; i < num_edges; i++)
+    if (edge_src[i] >= num_vertices || edge_dst[i] >= num_vertices) return 0;
+  ClusterPlan P;
+  if (build_cluster_plan(num_vertices, num_edges, edge_src, edge_dst, P)) return 0;
+  uint32_t w = 0;
+  auto put = [&](uint32_t x) {
+    if (out && w < cap) out[w] = x;
+    w++;
+  };
+  put(num_vertices);
+  uint32_t ncover = 0;
+  for (auto &p : P.passes) ncover += p.kind == ClusterPlan::kEncode;
+  put(ncover);
+  for (auto &p : P.passes)
+    if (p.kind == ClusterPlan::kEncode) put(P.clusters[p.arg].members[0]);
+  put((uint32_t)P.clusters.size());
+  for (auto &c : P.clusters) {
+    put(c.valid ? 1 : 0);
+    put((uint32_t)c.members.size());
+    for (uint32_t u : c.members) put(u);
+  }
+  put(P.final_cluster);
+  for (uint32_t u : P.final_pos) put(u);
+  put((uint32_t)P.passes.size());
+  for (auto &p : P.passes) {
+    put(p.kind);
+    put(p.arg);
+  }
+  return w;
+}
+
+int dsm_get_refined_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words) {
+  if (!e || !words || q >= e->queries.size() || u >= e->queries[q].n) return DSM_EINVAL;
+  HIPCHK(hipSetDevice(e->opt.device));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  const uint32_t r = e->qref[q][u];
+  HIPCHK(hipMemcpy(words, e->bm.p + (size_t)r * e->words, (size_t)e->words * 8, hipMemcpyDeviceToHost));
+  return DSM_OK;
+}
+
 int dsm_get_query_time(dsm_engine *e, double *ms, uint32_t num_queries) {
   if (!e || !ms || num_queries != e->queries.size()) return DSM_EINVAL;
   HIPCHK(hipSetDevice(e->opt.device));
@@ -1468,6 +1725,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     uint32_t wpb, bpc, dmin, lw;
     bool ls, seg;
     uint32_t dev, lb;
+    bool pl;
   };
   auto max_bpc = [&](uint32_t wpb) { return dfs_bpc(e, wpb); };
   // With overlap_apply, pipelined rounds run beside the next batch's update
@@ -1490,6 +1748,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     e->leaf_sets = c.ls;
     e->leaf_segments = c.seg;
     e->donate_every = c.dev;
+    e->bit_probe_last = c.pl;
     if (loaded) HIPCHK(hipEventRecord(e->ev_pre, e->stream));
     HIPCHK(launch_ctl_reset(nullptr, nullptr, e->tuple_count.p, e->counts.p, nq * 2, e->stream));
     int rc2 = run_round(e, rb.p, rb.p + k, k, 0, e->ev[9], e->ev[10], e->ev[11]);
@@ -1514,7 +1773,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     return DSM_OK;
   };
   Cfg best{e->dfs_wpb, e->dfs_blocks / (uint32_t)e->num_cu, e->donate_min, e->lane_work, e->leaf_sets,
-           e->leaf_segments, e->donate_every, e->leaf_budget};
+           e->leaf_segments, e->donate_every, e->leaf_budget, e->bit_probe_last};
   // warm-up: clocks and caches settle before anything is compared
   float best_ms = 1e30f;
   for (int r = 0; r < 4; r++) {
@@ -1611,6 +1870,12 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
       }
     }
   }
+  // 7. candidate bit before or after the membership tests
+  {
+    Cfg c = best;
+    c.pl = !best.pl;
+    if ((rc = consider(c))) return rc;
+  }
   // the kept configuration, timed once more
   best_ms = 1e30f;
   for (int r = 0; r < 2; r++) {
@@ -1627,6 +1892,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     rep->leaf_segments = best.seg ? 1 : 0;
     rep->donate_every = best.dev;
     rep->leaf_budget = best.lb;
+    rep->bit_probe_last = best.pl ? 1 : 0;
     rep->trials = trials;
     rep->ms_initial = ms0;
     rep->ms_best = best_ms;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace dsm {
 
@@ -159,6 +160,7 @@ struct DfsArgs {
   uint32_t leaf_segments;        // leaf lists scanned as one segmented array
   uint32_t poll_max;             // longest sleep between polls of a waiting wave (s_sleep 8 units)
   uint32_t leaf_sets;            // fused leaf level on/off
+  uint32_t bit_probe_last;       // candidate bit probed after the membership tests
   uint32_t nq;                   // queries (count layout)
   const uint32_t *seed_tag;      // per seed: count slot << 1 | sign
   unsigned long long watchdog;   // wall-clock ticks an idle wave may wait
@@ -198,6 +200,97 @@ struct DirtyArgs {
   uint32_t list_cap;
   ApplyCtl *actl;
 };
+
+// Message of the last failed call that has no engine (dsm_last_error(NULL)).
+void set_global_error(const char *fmt, ...);
+
+// Sorted-set intersection (dsm_intersect.hip): k_intersect writes chunk c's
+// members of B at hits[c * 64 ..] and their number at cnt[c].
+uint32_t intersect_chunks(uint32_t na);
+hipError_t launch_intersect(const uint32_t *A, uint32_t na, const uint32_t *B, uint32_t nb, uint32_t *hits,
+                            uint32_t *cnt, hipStream_t s);
+
+// ---- cluster-encoding refinement (dsm_cluster_plan.cpp, dsm_cluster.hip) ---
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kMaxClusters = 150;       // MAX_CLUSTERS, defs.h:22
+constexpr uint32_t kMaxClusterLayers = 100;  // MAX_LAYERS, defs.h:23
+constexpr uint32_t kMaxSmallClusters = 32;   // per combine (filter.cu:1056-1061)
+
+// Host plan of one query's refinement: clusters with their bit positions, the
+// combines, the passes in execution order and each query vertex's final
+// position (its refined candidate set).
+struct ClusterPlan {
+  struct Cluster {
+    std::vector<uint32_t> members;  // core first, then its neighbours
+    bool valid = true;
+    bool merged = false;            // made by a merge of clusters left and right
+    uint32_t left = kNone, right = kNone;
+    uint32_t left_pos = kNone, right_pos = kNone;  // the core's positions in left / right
+    uint32_t pos = 0;               // first bit position
+  };
+  struct Combine {
+    uint32_t big = kNone;           // the cluster that receives the combination
+    bool keep = false;              // big existed (AND into it) or was created (OR)
+    std::vector<uint32_t> small;
+  };
+  enum { kEncode = 0, kMerge = 1, kCombine = 2 };
+  struct Pass {
+    uint32_t kind, arg;             // encode / merge: cluster; combine: index into combines
+  };
+  std::vector<Cluster> clusters;
+  std::vector<Combine> combines;
+  std::vector<Pass> passes;
+  std::vector<uint32_t> final_pos;  // per query vertex
+  uint32_t final_cluster = 0;
+  uint32_t positions = 0;
+};
+
+// Returns 0, or -1 when the plan exceeds one of the reference's limits.
+int build_cluster_plan(uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *ev, ClusterPlan &out);
+
+// One pass of one query on the device. Bitmap rows index `bm` (signature rows
+// first, then every query's position rows); `alive` rows are the candidate
+// lists the passes prune (d_u_candidate_vs_ of the reference).
+enum : uint32_t { kOpEncode = 0, kOpMerge = 1, kOpCombineNew = 2, kOpCombineKeep = 3 };
+struct alignas(16) ClusterOp {
+  uint32_t kind;
+  uint32_t alive;       // alive row of the cluster's core
+  uint32_t core;        // bm row of the core position; targets at core+1 .. core+ntarget
+  uint32_t ntarget;
+  uint32_t check_off;   // a core candidate survives when all rows crow[check_off ..+ncheck) hold it
+  uint32_t ncheck;
+  uint32_t pad[2];
+  uint16_t toff[kMaxQ]; // target t: the AND of rows crow[toff[t] .. +tcnt[t]) at the neighbour
+  uint16_t tcnt[kMaxQ];
+  uint8_t tlab[kMaxQ];  // device label of target t's query vertex (every condition implies it)
+};
+
+struct ClusterHeavy {
+  uint32_t op, v, lo, pad;  // neighbours [lo, lo + kClusterChunk) of candidate v of op
+};
+constexpr uint32_t kClusterChunk = 4096;
+
+struct ClusterArgs {
+  DevGraph g;
+  uint64_t *bm;
+  uint32_t words;
+  uint64_t *alive;
+  const ClusterOp *ops;
+  uint32_t nops;
+  const uint32_t *crow;
+  ClusterHeavy *heavy;
+  uint32_t heavy_cap;
+  uint32_t *ctl;        // [2]: heavy entries queued, heavy entries claimed
+};
+
+// Zeroes bm rows [first_row, first_row + nrows), copies the listed (alive row,
+// bm row) pairs into the alive rows and zeroes nctl counters.
+hipError_t launch_cluster_init(uint64_t *bm, uint32_t words, uint32_t first_row, uint32_t nrows, uint64_t *alive,
+                               const uint32_t *alive_src, uint32_t nalive_src, uint32_t *ctl, uint32_t nctl,
+                               hipStream_t s);
+hipError_t launch_cluster_pass(const ClusterArgs &a, uint32_t max_blocks, hipStream_t s);
+hipError_t launch_cluster_heavy(const ClusterArgs &a, uint32_t blocks, hipStream_t s);
 
 // ---- launchers (dsm_kernels.hip) -------------------------------------------
 hipError_t launch_apply_merge(const DevGraph &g, const BatchDev &b, unsigned long long scratch_cap,

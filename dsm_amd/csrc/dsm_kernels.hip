@@ -19,6 +19,7 @@
 //   k_dfs                           persistent wave64 DFS with per-wave LDS
 //                                   stacks, work donation and shared leaf sets
 //                                   (gpu_match.cu:96-876 restated for CDNA)
+#include "dsm_device.h"
 #include "dsm_internal.h"
 
 namespace dsm {
@@ -27,62 +28,6 @@ namespace {
 constexpr int kApplyWaves = 4;
 constexpr uint32_t kApplyWave = 8192;  // longer lists are copied by the whole grid
 constexpr int kLight = 64;          // dirty vertices up to this degree: one lane each
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
-__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-__device__ __forceinline__ uint32_t bcast(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ uint64_t bcast64(uint64_t x) {
-  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
-  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <typename T>
-__device__ __forceinline__ T ld_relaxed(T *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void st_release(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ bool cas_u32(uint32_t *p, uint32_t expect, uint32_t desire) {
-  return __hip_atomic_compare_exchange_strong(p, &expect, desire, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// # elements < x in sorted a[0, n)
-__device__ __forceinline__ uint32_t lower_bound(const uint32_t *a, uint32_t n, uint32_t x) {
-  uint32_t lo = 0;
-  while (n > 0) {
-    uint32_t half = n >> 1;
-    if (a[lo + half] < x) {
-      lo += half + 1;
-      n -= half + 1;
-    } else {
-      n = half;
-    }
-  }
-  return lo;
-}
-
-__device__ __forceinline__ bool contains(const uint32_t *a, uint32_t n, uint32_t x) {
-  if (n == 0) return false;
-  uint32_t lo = 0;
-  while (n > 1) {
-    uint32_t half = n >> 1;
-    lo = (a[lo + half] <= x) ? lo + half : lo;
-    n -= half;
-  }
-  return a[lo] == x;
-}
 
 __device__ __forceinline__ bool bit_of(const uint64_t *bm, uint32_t words, uint32_t sig, uint32_t v) {
   return (bm[(size_t)sig * words + (v >> 6)] >> (v & 63)) & 1ull;
@@ -941,7 +886,6 @@ struct WaveHdr {
   uint32_t n, query, plan, tag;
   uint32_t fstate, fn;       // leaf set F of the current positions 0..n-3 (see leaf_count)
 };
-constexpr uint32_t kWin = 256;   // window entries staged in LDS per intersection
 constexpr uint32_t kFCap = 2048;  // largest leaf set F kept in LDS
 // One level above the leaf, the candidates of a 64-entry window count their
 // leaf images inline until their summed scan or search work passes
@@ -1031,59 +975,12 @@ __device__ __forceinline__ void init_level(WaveHdr &S, const DevGraph &g, uint32
   wave_sync();
 }
 
-// Lower bounds of key0 (lanes 0-31) and key1 (lanes 32-63) in sorted L[0, d),
-// by a 32-ary search per half-wave: every round probes 32 evenly spaced
-// entries of the remaining range (one load per lane) and keeps the segment
-// the key falls in, so a list of d entries takes about log32(d) + 1 dependent
-// loads instead of log2(d).
-__device__ __forceinline__ void bounds2(const uint32_t *L, uint32_t d, uint32_t key0, uint32_t key1,
-                                        uint32_t &r0, uint32_t &r1, uint32_t &loads) {
-  const uint32_t lane = lane_id();
-  const uint32_t k = lane & 31;
-  const uint32_t key = lane < 32 ? key0 : key1;
-  const uint64_t halfmask = lane < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
-  uint32_t a = 0, b = d;  // the bound lies in [a, b]
-  while (ballot(b > a)) {
-    const uint32_t m = b - a;
-    const uint32_t step = m > 32 ? (m + 31) / 32 : 1u;
-    const uint32_t p = a + k * step;
-    bool less = false;
-    if (p < b) {
-      less = L[p] < key;
-      loads++;
-    }
-    const uint32_t c = (uint32_t)__popcll(ballot(less) & halfmask);
-    if (b > a) {
-      if (c == 0) {
-        b = a;
-      } else {
-        const uint32_t na = a + (c - 1) * step + 1;
-        const uint32_t nb = a + c * step < b ? a + c * step : b;
-        a = na;
-        b = nb;
-      }
-    }
-  }
-  r0 = __builtin_amdgcn_readlane(a, 0);
-  r1 = __builtin_amdgcn_readlane(a, 32);
-}
-
-__device__ __forceinline__ bool lds_contains(const uint32_t *a, uint32_t n, uint32_t x) {
-  if (n == 0) return false;
-  uint32_t lo = 0;
-  while (n > 1) {
-    const uint32_t half = n >> 1;
-    lo = (a[lo + half] <= x) ? lo + half : lo;
-    n -= half;
-  }
-  return a[lo] == x;
-}
-
-// Tests source entries [p, min(p + 64, e)) of `src`, one per lane:
-// injectivity against the positions in `inj`, membership in the list of every
-// position in `oth`, then the candidate bit of signature `sig` for the
-// survivors only (the bitmap words are the scattered reads; the membership
-// windows are contiguous). The source chunk is sorted, so the entries still
+// Tests source entries [p, min(p + 64, e)) of `src`, one per lane: the
+// candidate bit of signature `sig`, injectivity against the positions in
+// `inj` and membership in the list of every position in `oth`. The bit is
+// probed first (it narrows the membership windows) or, with bit_probe_last,
+// only for the entries that passed the membership tests (the bitmap words are
+// scattered reads; the windows are contiguous). The source chunk is sorted, so the entries still
 // alive span [wmin, wmax]; the wave finds that window of the other list
 // cooperatively (bounds2) and, when it holds at most kWin entries, stages it
 // in LDS for the lanes' searches.
@@ -1095,7 +992,7 @@ __device__ __forceinline__ bool test_entries(const WaveMem &M, const DfsArgs &a,
   const uint32_t i = p + lane;
   bool ok = i < e;
   w = ok ? src[i] : 0xFFFFFFFFu;
-  ok = ok && bit_of(a.bm, a.words, sig, w);
+  if (!a.bit_probe_last) ok = ok && bit_of(a.bm, a.words, sig, w);
   while (inj) {
     const uint32_t j = __builtin_ctz(inj);
     inj &= inj - 1;
@@ -1104,26 +1001,10 @@ __device__ __forceinline__ bool test_entries(const WaveMem &M, const DfsArgs &a,
   while (oth) {
     const uint32_t j = __builtin_ctz(oth);
     oth &= oth - 1;
-    const uint64_t alive = ballot(ok);
-    if (alive == 0) break;
-    const uint32_t wmin = __shfl(w, (int)__builtin_ctzll(alive));
-    const uint32_t wmax = __shfl(w, (int)(63 - __builtin_clzll(alive)));
-    const uint32_t *L = a.g.pool + S.fs[j];
-    uint32_t lo, hi;
-    bounds2(L, S.fd[j], wmin, wmax + 1, lo, hi, loads);
-    const uint32_t W = hi - lo;
-    if (W <= kWin) {
-      for (uint32_t k = lane; k < W; k += 64) M.win[k] = L[lo + k];
-      wave_sync();
-      ok = ok && lds_contains(M.win, W, w);
-      wave_sync();
-    } else {
-      if (ok) {
-        loads += 32u - (uint32_t)__builtin_clz(W);
-        ok = contains(L + lo, W, w);
-      }
-    }
+    if (ballot(ok) == 0) break;
+    ok = window_member(a.g.pool + S.fs[j], S.fd[j], w, ok, M.win, loads);
   }
+  if (a.bit_probe_last) ok = ok && bit_of(a.bm, a.words, sig, w);
   return ok;
 }
 

@@ -38,6 +38,7 @@ EXPORTS = (
     "dsm_file_graph", "dsm_file_query", "dsm_file_num_batches", "dsm_file_batch", "dsm_file_error",
     "dsm_server_create", "dsm_server_submit", "dsm_server_wait", "dsm_server_error", "dsm_server_destroy",
     "dsm_admit", "dsm_estimate_request_bytes", "dsm_autotune", "dsm_process_batches",
+    "dsm_get_refined_candidates", "dsm_cluster_plan", "dsm_intersect_sorted",
 )
 
 
@@ -68,14 +69,16 @@ class Options(C.Structure):
                 ("donate_min", C.c_uint32), ("queue_capacity", C.c_uint32), ("watchdog_ms", C.c_uint32),
                 ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("donate_every", C.c_uint32),
                 ("donate_min_n2", C.c_uint32), ("poll_max", C.c_uint32), ("leaf_segments", C.c_int),
-                ("graph_rounds", C.c_int), ("overlap_apply", C.c_int), ("leaf_budget", C.c_uint32)]
+                ("graph_rounds", C.c_int), ("overlap_apply", C.c_int), ("leaf_budget", C.c_uint32),
+                ("cluster_filter", C.c_int), ("bit_probe_last", C.c_int)]
 
 
 class TuneReport(C.Structure):
     _fields_ = [("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("donate_min", C.c_uint32),
                 ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("leaf_segments", C.c_int),
                 ("donate_every", C.c_uint32), ("trials", C.c_uint32),
-                ("ms_initial", C.c_float), ("ms_best", C.c_float), ("leaf_budget", C.c_uint32)]
+                ("ms_initial", C.c_float), ("ms_best", C.c_float), ("leaf_budget", C.c_uint32),
+                ("bit_probe_last", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -90,7 +93,8 @@ class Stats(C.Structure):
                 ("pool_entries_used", C.c_uint64), ("pool_entries_capacity", C.c_uint64),
                 ("dfs_entries", C.c_uint64), ("dfs_search_loads", C.c_uint64), ("dfs_launches", C.c_uint32),
                 ("dfs_merge_bytes", C.c_uint64), ("dfs_busy_ms", C.c_double), ("dfs_waves", C.c_uint64),
-                ("dfs_block_imbalance", C.c_double), ("dfs_head_idle", C.c_double), ("dfs_tail_idle", C.c_double)]
+                ("dfs_block_imbalance", C.c_double), ("dfs_head_idle", C.c_double), ("dfs_tail_idle", C.c_double),
+                ("ms_refine", C.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -145,6 +149,12 @@ def lib():
     L.dsm_get_graph.argtypes = [vp, vp, vp, C.c_uint64, vp, C.POINTER(C.c_uint64)]
     L.dsm_get_candidates.restype = C.c_int
     L.dsm_get_candidates.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
+    L.dsm_get_refined_candidates.restype = C.c_int
+    L.dsm_get_refined_candidates.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
+    L.dsm_intersect_sorted.restype = C.c_int
+    L.dsm_intersect_sorted.argtypes = [C.c_int, vp, C.c_uint32, vp, C.c_uint32, vp, C.POINTER(C.c_uint32)]
+    L.dsm_cluster_plan.restype = C.c_uint32
+    L.dsm_cluster_plan.argtypes = [C.c_uint32, C.c_uint32, vp, vp, vp, C.c_uint32]
     L.dsm_get_query_time.restype = C.c_int
     L.dsm_get_query_time.argtypes = [vp, vp, C.c_uint32]
     L.dsm_num_vertices.restype = C.c_uint32
@@ -253,7 +263,7 @@ class Engine:
                  tuples_in_host_memory=False, waves_per_block=4, blocks_per_cu=0, donate_min=0,
                  queue_capacity=0, watchdog_ms=0, lane_work=0, leaf_sets=True, graph_rounds=True,
                  donate_every=0, donate_min_n2=0, leaf_segments=True, poll_max=0, overlap_apply=True,
-                 leaf_budget=0):
+                 leaf_budget=0, cluster_filter=False, bit_probe_last=False):
         L = lib()
         n, lab, eu, ev = graph
         self._keep = []
@@ -286,6 +296,8 @@ class Engine:
         opt.poll_max = poll_max
         opt.overlap_apply = int(bool(overlap_apply))
         opt.leaf_budget = leaf_budget
+        opt.cluster_filter = int(bool(cluster_filter))
+        opt.bit_probe_last = int(bool(bit_probe_last))
         h = C.c_void_p()
         rc = L.dsm_create(C.byref(gd), qds, len(queries), C.byref(opt), C.byref(h))
         self._keep = None
@@ -391,11 +403,43 @@ class Engine:
         bits = np.unpackbits(words.view(np.uint8), bitorder="little")
         return bits[: self.n].astype(bool)
 
+    def refined_candidates(self, q, u):
+        """The candidates the seeding and the DFS use for query vertex u of
+        query q (the cluster-encoding refinement with cluster_filter, else the
+        LDF/NLF set), as a bool array over vertices."""
+        words = np.zeros((self.n + 63) // 64, np.uint64)
+        self._check(lib().dsm_get_refined_candidates(self._h, q, u, words.ctypes.data))
+        bits = np.unpackbits(words.view(np.uint8), bitorder="little")
+        return bits[: self.n].astype(bool)
+
     def query_time(self):
         """DFS task time per query in ms since the last call (resets it)."""
         ms = np.zeros(self.nq, np.float64)
         self._check(lib().dsm_get_query_time(self._h, ms.ctypes.data, self.nq))
         return ms
+
+
+def cluster_plan_words(n, edges):
+    """The engine's cluster plan of a query (dsm_cluster_plan) as a word
+    array, or None past the reference's limits."""
+    eu = _u32([a for a, _ in edges])
+    ev = _u32([b for _, b in edges])
+    buf = np.zeros(1 << 16, np.uint32)
+    w = lib().dsm_cluster_plan(int(n), len(eu), eu.ctypes.data, ev.ctypes.data, buf.ctypes.data, buf.shape[0])
+    return buf[:w].copy() if w else None
+
+
+def intersect_sorted(a, b, device=0):
+    """a ∩ b on the device (dsm_intersect_sorted); a and b strictly ascending."""
+    L = lib()
+    a, b = _u32(a), _u32(b)
+    out = np.zeros(max(1, min(len(a), len(b))), np.uint32)
+    n = C.c_uint32()
+    rc = L.dsm_intersect_sorted(device, a.ctypes.data, len(a), b.ctypes.data, len(b), out.ctypes.data,
+                                C.byref(n))
+    if rc != DSM_OK:
+        raise DsmError(rc, L.dsm_last_error(None).decode())
+    return out[: n.value].copy()
 
 
 def admit(request_bytes, in_flight_bytes, budget):

@@ -104,6 +104,15 @@ typedef struct {
                                 entries) of a 64-candidate window before a candidate
                                 descends so that its scan can be shared with idle waves
                                 (0 = 2048; 0xFFFFFFFF = never) */
+  int cluster_filter;        /* 1: after every update the candidates are refined by the
+                                cluster encoding of the reference's clusterFilter
+                                (filter.cu:1525-1654: spanning tree, vertex cover,
+                                clusters, encode/merge/combine passes); the seeding and
+                                the DFS then use the refined sets. Counts are the same
+                                either way; 0: LDF/NLF candidates only */
+  int bit_probe_last;        /* 1: the DFS probes a source entry's candidate bit after its
+                                membership tests instead of before them (searched by
+                                dsm_autotune) */
 } dsm_options;
 
 /* Per-batch statistics of the last dsm_apply_batch + dsm_match_batch. */
@@ -135,6 +144,7 @@ typedef struct {
    * and after its last one (the tail) */
   double dfs_head_idle;
   double dfs_tail_idle;
+  float ms_refine;                 /* device time of the cluster-encoding refinement */
 } dsm_stats;
 
 /* Outcome of dsm_autotune: the launch parameters kept and the DFS time of the
@@ -146,6 +156,7 @@ typedef struct {
   uint32_t trials;        /* configurations timed */
   float ms_initial, ms_best;
   uint32_t leaf_budget;
+  int bit_probe_last;
 } dsm_tune_report;
 
 /* ------------------------------------------------------------ engine life */
@@ -162,8 +173,9 @@ int dsm_create(const dsm_graph_desc *graph, const dsm_query_desc *queries, uint3
 
 void dsm_destroy(dsm_engine *e);
 
-/* Last error message of the engine (or of the last failed dsm_create when e is
- * NULL). Never NULL. */
+/* Last error message of the engine (or, when e is NULL, of the last failed
+ * call without an engine on this thread: dsm_create, dsm_intersect_sorted).
+ * Never NULL. */
 const char *dsm_last_error(const dsm_engine *e);
 
 /* --------------------------------------------------------- per-batch path */
@@ -230,11 +242,44 @@ int dsm_get_graph(dsm_engine *e, uint64_t *offsets, uint32_t *neighbours, uint64
  * reference keeps the equivalent in d_compact_encodings_ (main.cu:141-142). */
 int dsm_get_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words);
 
+/* Candidates the seeding and the DFS use for query vertex u of query q, in
+ * the layout of dsm_get_candidates: with cluster_filter, the vertex's bits at
+ * its final position of the cluster encoding (enc_pos_of_u_, filter.cu:
+ * 1589-1609, read out as compact does, filter.cu:1692-1750); without it, the
+ * LDF/NLF set. */
+int dsm_get_refined_candidates(dsm_engine *e, uint32_t q, uint32_t u, uint64_t *words);
+
+/* The cluster plan the refinement uses for a query (pure host logic): the
+ * spanning tree's vertex cover (decycle.cpp:6-135, filter.cu:23-116), the
+ * clusters with their merges and combines (filter.cu:131-392), the passes in
+ * the order encode() runs them (filter.cu:936-1177) and each vertex's final
+ * position (filter.cu:1589-1609), as words: n, ncover, cover[ncover],
+ * nclusters, per cluster (valid, size, members[size]), final cluster,
+ * position[n], npasses, per pass (kind 0 encode / 1 merge / 2 combine, cluster
+ * or combine index). Edge lines in file order. Returns the number of words
+ * (written up to cap), or 0 for an invalid query or one past the reference's
+ * limits (150 clusters, 32 small clusters per combine). No counterpart in
+ * the reference, which keeps the plan in host structures (structure.cuh). */
+uint32_t dsm_cluster_plan(uint32_t num_vertices, uint32_t num_edges, const uint32_t *edge_src,
+                          const uint32_t *edge_dst, uint32_t *out, uint32_t cap);
+
 /* DFS task time per query (ms, summed over the waves, so it can exceed the
  * wall time) since the engine was created or this was last called; the call
  * resets it. No counterpart in the reference, which runs one query per
  * process; it tells which standing queries cost the most. */
 int dsm_get_query_time(dsm_engine *e, double *ms, uint32_t num_queries);
+
+/* Sorted-set intersection a ∩ b on device `device`, with the membership test
+ * the DFS uses for every backward list (a cooperative wave search for the
+ * window the 64 lanes' values span, then LDS-staged binary searches): the
+ * reference's intersect (gpu_match.cu:21-94), which binary-searches each
+ * element of the first set in the second. a and b are host arrays, strictly
+ * ascending, with no value 0xFFFFFFFF (else DSM_EINVAL); out (room for
+ * min(na, nb) values) receives the intersection in ascending order and
+ * *num_out its size. Allocates and frees its device buffers; errors are read
+ * with dsm_last_error(NULL). */
+int dsm_intersect_sorted(int device, const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out,
+                         uint32_t *num_out);
 
 /* Number of vertices and queries of the engine. */
 uint32_t dsm_num_vertices(const dsm_engine *e);

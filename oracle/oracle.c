@@ -114,6 +114,8 @@ og_t *og_new(uint32_t n, const uint32_t *labels, uint64_t m,
         g->adj[eu[i]][g->deg[eu[i]]++] = ev[i];
         g->adj[ev[i]][g->deg[ev[i]]++] = eu[i];
     }
+    /* lists are independent: sorted on the host's threads */
+#pragma omp parallel for schedule(dynamic, 4096)
     for (uint32_t v = 0; v < n; v++) {
         qsort(g->adj[v], g->deg[v], 4, cmp_u32);
         uint32_t w = 0;
@@ -449,6 +451,411 @@ uint64_t og_brute_total(const og_t *g, const oq_t *q, uint32_t *sink, uint64_t s
     for (uint32_t l = 0; l < nl; l++) free(b.bylab[l]);
     free(b.bylab); free(b.nbylab);
     return b.count;
+}
+
+/* ------------------------------------------------- cluster-encoding filter */
+
+/* Restatement of the reference's cluster-encoding candidate refinement
+ * (clusterFilter, filter.cu:1525-1654): the query is reduced to a BFS spanning
+ * tree (decycle, decycle.cpp:6-135), a minimum vertex cover of the tree is
+ * chosen (getVertexCover / dfs_cvc, filter.cu:23-116), each cover vertex and
+ * its tree neighbours form a cluster, clusters sharing a vertex are merged
+ * layer by layer and clusters with the same core are combined
+ * (clustering, filter.cu:131-392; encodingMeta::init, structure.cu:107-135).
+ * Every cluster member owns one bit position; encodeKernel, mergeKernel and
+ * combineMultipleClustersKernel (filter.cu:615-934), run in the order of
+ * encode() (filter.cu:936-1177), set and clear those bits per data vertex, and
+ * a query vertex's refined candidates are the bits of its position in the
+ * last valid cluster that holds it (filter.cu:1589-1609; compact,
+ * filter.cu:1692-1750).
+ *
+ * The plan uses the query's adjacency in edge-line order (readGraphToCPU-
+ * Dynamic appends neighbours as the e lines come, io.cu:176-196), which fixes
+ * the BFS tree and the cover's order. One deviation: the reference updates a
+ * vertex's own core bit with a plain read-modify-write of its 32-bit word
+ * (filter.cu:765, :876-884) while other warps may atomically update other bits
+ * of the same word; a lost update there would make its result depend on
+ * timing. Here every bit update takes effect. */
+
+#define OC_MAXC 150 /* MAX_CLUSTERS, defs.h:22 */
+#define OC_MAXL 100 /* MAX_LAYERS, defs.h:23 */
+#define OC_MAXSMALL 32 /* small clusters per combine (filter.cu:1056-1061) */
+
+enum { OC_ENCODE = 0, OC_MERGE = 1, OC_COMBINE = 2 };
+
+typedef struct {
+    uint32_t n;
+    uint32_t tdeg[OQ_MAXV], tadj[OQ_MAXV][OQ_MAXV]; /* spanning tree, edge-line order */
+    uint32_t ncover, cover[OQ_MAXV];
+    uint32_t nc;                                    /* clusters, valid or not */
+    uint32_t csz[OC_MAXC], cu[OC_MAXC][OQ_MAXV + 1];
+    uint8_t valid[OC_MAXC];
+    uint32_t coff[OC_MAXC + 1];                     /* first bit position of each cluster */
+    uint32_t nlayers, per_layer[OC_MAXL + 2], layer_off[OC_MAXL + 2];
+    int checkpoint[OC_MAXL + 2];
+    uint32_t nmerge, mleft[OC_MAXC], mright[OC_MAXC];
+    uint32_t ncomb, cbig[OC_MAXC], ctype[OC_MAXC], cnsmall[OC_MAXC], csmall[OC_MAXC][OC_MAXSMALL];
+    uint32_t final_cluster;
+    uint32_t enc_pos[OQ_MAXV];
+    /* the device passes in execution order */
+    uint32_t nops, op_kind[3 * OC_MAXC], op_arg[3 * OC_MAXC], op_arg2[3 * OC_MAXC];
+} ocp_t;
+
+/* decycle.cpp:6-135: root = the first vertex of highest degree; BFS in
+ * adjacency order keeps the edge to each newly reached vertex. */
+static void oc_decycle(ocp_t *p, uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *ev) {
+    uint32_t deg[OQ_MAXV] = {0}, adj[OQ_MAXV][2 * OQ_MAXV], eid[OQ_MAXV][2 * OQ_MAXV];
+    uint8_t keep[2 * OQ_MAXV * OQ_MAXV];
+    memset(keep, 0, sizeof(keep));
+    for (uint32_t i = 0; i < m; i++) {
+        adj[eu[i]][deg[eu[i]]] = ev[i]; eid[eu[i]][deg[eu[i]]++] = i;
+        adj[ev[i]][deg[ev[i]]] = eu[i]; eid[ev[i]][deg[ev[i]]++] = i;
+    }
+    uint32_t root = 0;
+    for (uint32_t u = 1; u < n; u++) if (deg[u] > deg[root]) root = u;
+    uint32_t qu[OQ_MAXV], qh = 0, qt = 0, seen = 1u << root;
+    qu[qt++] = root;
+    while (qh < qt) {
+        uint32_t t = qu[qh++];
+        for (uint32_t k = 0; k < deg[t]; k++) {
+            uint32_t w = adj[t][k];
+            if (seen >> w & 1) continue;
+            seen |= 1u << w;
+            qu[qt++] = w;
+            keep[eid[t][k]] = 1;
+        }
+    }
+    p->n = n;
+    for (uint32_t u = 0; u < n; u++) {
+        p->tdeg[u] = 0;
+        for (uint32_t k = 0; k < deg[u]; k++)
+            if (keep[eid[u][k]]) p->tadj[u][p->tdeg[u]++] = adj[u][k];
+    }
+}
+
+static void oc_cvc(const ocp_t *p, uint32_t v, uint32_t father, int f[][2]) {
+    f[v][0] = 0; f[v][1] = 1;
+    for (uint32_t k = 0; k < p->tdeg[v]; k++) {
+        uint32_t c = p->tadj[v][k];
+        if (c == father) continue;
+        oc_cvc(p, c, v, f);
+        f[v][0] += f[c][1];
+        f[v][1] += f[c][0] < f[c][1] ? f[c][0] : f[c][1];
+    }
+}
+
+/* filter.cu:23-116: tree DP, then a top-down pass with an explicit stack
+ * (children pushed in adjacency order, so popped last first). */
+static void oc_cover(ocp_t *p) {
+    int f[OQ_MAXV][2];
+    uint32_t root = 0;
+    for (uint32_t u = 1; u < p->n; u++) if (p->tdeg[u] > p->tdeg[root]) root = u;
+    oc_cvc(p, root, 0xFFFFFFFFu, f);
+    uint32_t st[OQ_MAXV], sf[OQ_MAXV], ssel[OQ_MAXV], sp = 0;
+    st[sp] = root; sf[sp] = 0xFFFFFFFFu; ssel[sp] = 0; sp++;
+    p->ncover = 0;
+    while (sp) {
+        sp--;
+        uint32_t cur = st[sp], fa = sf[sp], fsel = ssel[sp], child_sel;
+        if (f[cur][1] <= f[cur][0]) {
+            p->cover[p->ncover++] = cur;
+            child_sel = 1;
+        } else if (fsel || fa == 0xFFFFFFFFu) {
+            child_sel = 0;
+        } else {
+            p->cover[p->ncover++] = cur;
+            child_sel = 1;
+        }
+        for (uint32_t k = 0; k < p->tdeg[cur]; k++) {
+            uint32_t c = p->tadj[cur][k];
+            if (c == fa) continue;
+            st[sp] = c; sf[sp] = cur; ssel[sp] = child_sel; sp++;
+        }
+    }
+}
+
+/* filter.cu:131-392: returns 0, or -1 when a limit of the reference
+ * (MAX_CLUSTERS, MAX_LAYERS, 32 small clusters per combine) is exceeded. */
+static int oc_clustering(ocp_t *p) {
+    uint32_t nc = p->ncover;
+    if (nc > OC_MAXC) return -1;
+    memset(p->valid, 1, sizeof(p->valid));
+    for (uint32_t c = 0; c < nc; c++) {
+        uint32_t core = p->cover[c];
+        p->csz[c] = p->tdeg[core] + 1;
+        p->cu[c][0] = core;
+        for (uint32_t k = 0; k < p->tdeg[core]; k++) p->cu[c][k + 1] = p->tadj[core][k];
+    }
+    for (uint32_t l = 0; l < OC_MAXL + 2; l++) { p->per_layer[l] = 0; p->layer_off[l] = 0; p->checkpoint[l] = -1; }
+    p->per_layer[0] = nc;
+    p->nmerge = p->ncomb = 0;
+    p->final_cluster = 0;
+    uint32_t layer = 0, i = 0, actual;
+    do {
+        uint32_t nnew = 0;
+        for (; i < nc; i++) {
+            if (!p->valid[i]) continue;
+            uint32_t core_i = p->cu[i][0];
+            for (uint32_t j = i + 1; j < nc; j++) {
+                if (!p->valid[j]) continue;
+                uint32_t core_j = p->cu[j][0];
+                for (uint32_t a = 1; a < p->csz[i]; a++)
+                    for (uint32_t b = 0; b < p->csz[j]; b++) {
+                        if (p->cu[i][a] != p->cu[j][b]) continue;
+                        uint32_t x = nc + nnew++;
+                        if (x >= OC_MAXC) return -1;
+                        p->final_cluster = x;
+                        p->valid[x] = 1;
+                        p->cu[x][0] = p->cu[i][a];
+                        p->cu[x][1] = core_i;
+                        if (b && core_i != core_j) { p->cu[x][2] = core_j; p->csz[x] = 3; }
+                        else p->csz[x] = 2;
+                        p->mleft[p->nmerge] = i;
+                        p->mright[p->nmerge] = j;
+                        p->nmerge++;
+                    }
+            }
+        }
+        layer++;
+        if (layer > OC_MAXL) return -1;
+        actual = nnew;
+        p->checkpoint[layer] = (int)(nc + nnew);
+        for (uint32_t o = nc; o < nc + nnew; o++) {
+            if (!p->valid[o]) continue;
+            uint32_t core = p->cu[o][0];
+            uint32_t members[OC_MAXC], nm = 0, nbr = 0; /* nbr: set of non-core vertices */
+            members[nm++] = o;
+            for (uint32_t k = 1; k < p->csz[o]; k++) nbr |= 1u << p->cu[o][k];
+            for (uint32_t x = o + 1; x < nc + nnew; x++) {
+                if (!p->valid[x] || p->cu[x][0] != core) continue;
+                members[nm++] = x;
+                for (uint32_t k = 1; k < p->csz[x]; k++) nbr |= 1u << p->cu[x][k];
+            }
+            if (nm == 1) continue;
+            uint32_t want = (uint32_t)__builtin_popcount(nbr) + 1, big = 0xFFFFFFFFu;
+            for (uint32_t k = 0; k < nm; k++) /* members are in ascending order */
+                if (p->csz[members[k]] == want) { big = members[k]; break; }
+            uint32_t ci = p->ncomb;
+            p->cnsmall[ci] = 0;
+            if (big != 0xFFFFFFFFu) {
+                p->ctype[ci] = 1;
+                for (uint32_t k = 0; k < nm; k++)
+                    if (members[k] != big) {
+                        if (p->cnsmall[ci] >= OC_MAXSMALL) return -1;
+                        p->csmall[ci][p->cnsmall[ci]++] = members[k];
+                    }
+            } else {
+                p->ctype[ci] = 0;
+                big = nc + nnew++;
+                actual++;
+                if (big >= OC_MAXC) return -1;
+                p->valid[big] = 1;
+                p->cu[big][0] = core;
+                p->csz[big] = 1;
+                for (uint32_t u = 0; u < p->n; u++) if (nbr >> u & 1) p->cu[big][p->csz[big]++] = u;
+                for (uint32_t k = 0; k < nm; k++) {
+                    if (p->cnsmall[ci] >= OC_MAXSMALL) return -1;
+                    p->csmall[ci][p->cnsmall[ci]++] = members[k];
+                }
+            }
+            p->final_cluster = big;
+            p->cbig[ci] = big;
+            actual -= p->cnsmall[ci];
+            for (uint32_t k = 0; k < p->cnsmall[ci]; k++) p->valid[p->csmall[ci][k]] = 0;
+            p->ncomb++;
+        }
+        nc += nnew;
+        p->per_layer[layer] = nnew;
+    } while (actual > 1);
+    p->nc = nc;
+    for (uint32_t l = 1; l <= layer; l++) p->layer_off[l] = p->layer_off[l - 1] + p->per_layer[l - 1];
+    p->layer_off[layer + 1] = nc;
+    p->nlayers = layer + 1;
+    p->coff[0] = 0;
+    for (uint32_t c = 0; c < nc; c++) p->coff[c + 1] = p->coff[c] + p->csz[c];
+    return 0;
+}
+
+/* encode(), filter.cu:936-1177: the order in which the passes run, including
+ * where each layer's combines fall (its combine checkpoint, or a later one, or
+ * the final pass). */
+static void oc_schedule(ocp_t *p) {
+    p->nops = 0;
+    for (uint32_t c = 0; c < p->per_layer[0]; c++) {
+        p->op_kind[p->nops] = OC_ENCODE; p->op_arg[p->nops++] = c;
+    }
+    uint32_t layer = 1, sum = p->per_layer[0], cptr = 0, mptr = 0;
+    for (uint32_t ci = p->per_layer[0]; ci < p->nc; ci++) {
+        if (ci == sum + p->per_layer[layer]) { sum += p->per_layer[layer]; layer++; }
+        if ((int)ci == p->checkpoint[layer]) {
+            uint32_t next = p->layer_off[layer + 1];
+            while (cptr < p->ncomb && p->cbig[cptr] < next) {
+                p->op_kind[p->nops] = OC_COMBINE; p->op_arg[p->nops++] = cptr++;
+            }
+            if (ci != next) { ci = next - 1; continue; }
+        }
+        p->op_kind[p->nops] = OC_MERGE; p->op_arg[p->nops] = ci; p->op_arg2[p->nops++] = mptr++;
+    }
+    if (p->checkpoint[p->nlayers - 1] == (int)p->nc)
+        while (cptr < p->ncomb) { p->op_kind[p->nops] = OC_COMBINE; p->op_arg[p->nops++] = cptr++; }
+    /* filter.cu:1589-1609: each vertex's first position, scanning the valid
+     * clusters down from the final one */
+    uint32_t seen = 0, cnt = 0;
+    for (int c = (int)p->final_cluster; c >= 0; c--) {
+        if (!p->valid[c]) continue;
+        if (cnt == p->n) break;
+        for (uint32_t k = 0; k < p->csz[c]; k++) {
+            uint32_t u = p->cu[c][k];
+            if (!(seen >> u & 1)) { seen |= 1u << u; p->enc_pos[u] = p->coff[c] + k; cnt++; }
+        }
+    }
+}
+
+static int oc_plan(ocp_t *p, uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *ev) {
+    if (n < 2 || n > 16) return -1;
+    memset(p, 0, sizeof(*p));
+    oc_decycle(p, n, m, eu, ev);
+    oc_cover(p);
+    if (oc_clustering(p)) return -1;
+    oc_schedule(p);
+    return 0;
+}
+
+/* The plan as flat words for the tests: n, ncover, cover[ncover], nc, then per
+ * cluster (valid, size, members...), final cluster, enc_pos[n], nops, then
+ * (kind, arg) per pass. Returns the word count (0 on a limit error). */
+uint32_t oq_cluster_plan(uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *ev, uint32_t *out,
+                         uint32_t cap) {
+    ocp_t *p = (ocp_t *)calloc(1, sizeof(ocp_t));
+    uint32_t w = 0;
+#define PUT(x) do { if (w < cap) out[w] = (x); w++; } while (0)
+    if (oc_plan(p, n, m, eu, ev) == 0) {
+        PUT(p->n); PUT(p->ncover);
+        for (uint32_t k = 0; k < p->ncover; k++) PUT(p->cover[k]);
+        PUT(p->nc);
+        for (uint32_t c = 0; c < p->nc; c++) {
+            PUT(p->valid[c]); PUT(p->csz[c]);
+            for (uint32_t k = 0; k < p->csz[c]; k++) PUT(p->cu[c][k]);
+        }
+        PUT(p->final_cluster);
+        for (uint32_t u = 0; u < p->n; u++) PUT(p->enc_pos[u]);
+        PUT(p->nops);
+        for (uint32_t k = 0; k < p->nops; k++) { PUT(p->op_kind[k]); PUT(p->op_arg[k]); }
+    }
+#undef PUT
+    free(p);
+    return w;
+}
+
+typedef struct {
+    const og_t *g;
+    uint64_t words;
+    uint64_t *bits; /* [positions][words] */
+} oc_enc_t;
+
+static inline int oc_get(const oc_enc_t *e, uint32_t pos, uint32_t v) {
+    return (int)(e->bits[(uint64_t)pos * e->words + (v >> 6)] >> (v & 63) & 1);
+}
+static inline void oc_set(oc_enc_t *e, uint32_t pos, uint32_t v) {
+    e->bits[(uint64_t)pos * e->words + (v >> 6)] |= 1ull << (v & 63);
+}
+static inline void oc_clr(oc_enc_t *e, uint32_t pos, uint32_t v) {
+    e->bits[(uint64_t)pos * e->words + (v >> 6)] &= ~(1ull << (v & 63));
+}
+
+/* First position at or after `from` that holds query vertex u (the scans of
+ * filter.cu:1104-1110 over query_us_compact_). */
+static uint32_t oc_find(const ocp_t *p, uint32_t from_cluster, uint32_t u) {
+    for (uint32_t c = from_cluster; c < p->nc; c++)
+        for (uint32_t k = 0; k < p->csz[c]; k++)
+            if (p->cu[c][k] == u) return p->coff[c] + k;
+    return 0xFFFFFFFFu;
+}
+
+/* Refined candidates of the query (edge lines eu/ev in file order) in graph
+ * g, from the LDF/NLF candidates `ldf` (qn rows of g->n bytes, og_ldf_nlf of
+ * the same query): out receives qn rows of g->n bytes. Returns 0, or -1 when
+ * the plan exceeds a limit of the reference. */
+int og_cluster_filter(const og_t *g, uint32_t qn, uint32_t qm, const uint32_t *eu, const uint32_t *ev,
+                      const uint8_t *ldf, uint8_t *out) {
+    ocp_t *p = (ocp_t *)calloc(1, sizeof(ocp_t));
+    if (oc_plan(p, qn, qm, eu, ev)) { free(p); return -1; }
+    uint32_t n = g->n;
+    oc_enc_t e;
+    e.g = g;
+    e.words = ((uint64_t)n + 63) / 64;
+    e.bits = (uint64_t *)calloc((size_t)p->coff[p->nc] * e.words + 1, 8);
+    /* candidate lists (d_u_candidate_vs_): LDF/NLF, entries removed by the
+     * merge and combine passes */
+    uint8_t *alive = (uint8_t *)malloc((size_t)qn * n + 1);
+    memcpy(alive, ldf, (size_t)qn * n);
+    uint32_t *snap = (uint32_t *)malloc((size_t)n * 4 + 4);
+    for (uint32_t k = 0; k < p->nops; k++) {
+        uint32_t kind = p->op_kind[k], arg = p->op_arg[k];
+        uint32_t c = kind == OC_COMBINE ? p->cbig[arg] : arg;
+        uint32_t core = p->cu[c][0], pos = p->coff[c], ns = 0;
+        uint8_t *al = alive + (size_t)core * n;
+        /* the list as the pass starts: a pass removes entries but still
+         * visits the neighbours of every entry it started with */
+        for (uint32_t v = 0; v < n; v++) if (al[v]) snap[ns++] = v;
+        if (kind == OC_ENCODE) { /* encodeKernel, filter.cu:615-717 */
+            for (uint32_t s = 0; s < ns; s++) {
+                uint32_t v = snap[s];
+                oc_set(&e, pos, v);
+                for (uint32_t a = 0; a < g->deg[v]; a++) {
+                    uint32_t w = g->adj[v][a];
+                    for (uint32_t i = 1; i < p->csz[c]; i++)
+                        if (ldf[(size_t)p->cu[c][i] * n + w]) oc_set(&e, pos + i, w);
+                }
+            }
+        } else if (kind == OC_MERGE) { /* mergeKernel, filter.cu:719-803; filter.cu:1101-1127 */
+            uint32_t left = p->mleft[p->op_arg2[k]], right = p->mright[p->op_arg2[k]];
+            uint32_t lpos = oc_find(p, left, core), rpos = oc_find(p, right, core);
+            uint32_t lcore = p->coff[left], rcore = p->coff[right];
+            for (uint32_t s = 0; s < ns; s++) {
+                uint32_t v = snap[s];
+                if (oc_get(&e, lpos, v) && oc_get(&e, rpos, v)) oc_set(&e, pos, v);
+                else al[v] = 0;
+                for (uint32_t a = 0; a < g->deg[v]; a++) {
+                    uint32_t w = g->adj[v][a];
+                    if (oc_get(&e, lcore, w)) oc_set(&e, pos + 1, w);
+                    if (rcore != rpos && oc_get(&e, rcore, w)) oc_set(&e, pos + 2, w);
+                }
+            }
+        } else { /* combineMultipleClustersKernel, filter.cu:805-934 */
+            uint32_t type = p->ctype[arg], nsm = p->cnsmall[arg];
+            const uint32_t *sm = p->csmall[arg];
+            /* per target position of the big cluster: the position of the same
+             * query vertex in each small cluster that holds it (non-core part) */
+            uint32_t tpos[OQ_MAXV + 1][OC_MAXSMALL], ntp[OQ_MAXV + 1];
+            for (uint32_t i = 1; i < p->csz[c]; i++) {
+                ntp[i] = 0;
+                for (uint32_t j = 0; j < nsm; j++)
+                    for (uint32_t x = 1; x < p->csz[sm[j]]; x++)
+                        if (p->cu[sm[j]][x] == p->cu[c][i]) { tpos[i][ntp[i]++] = p->coff[sm[j]] + x; break; }
+            }
+            for (uint32_t s = 0; s < ns; s++) {
+                uint32_t v = snap[s];
+                int all = 1;
+                for (uint32_t j = 0; j < nsm; j++) all &= oc_get(&e, p->coff[sm[j]], v);
+                if (!all) { oc_clr(&e, pos, v); al[v] = 0; }
+                else if (type == 0) oc_set(&e, pos, v);
+                for (uint32_t a = 0; a < g->deg[v]; a++) {
+                    uint32_t w = g->adj[v][a];
+                    for (uint32_t i = 1; i < p->csz[c]; i++) {
+                        int fin = 1;
+                        for (uint32_t t = 0; t < ntp[i]; t++) fin &= oc_get(&e, tpos[i][t], w);
+                        if (type == 1 && !fin) oc_clr(&e, pos + i, w);
+                        if (type == 0 && fin) oc_set(&e, pos + i, w);
+                    }
+                }
+            }
+        }
+    }
+    for (uint32_t u = 0; u < qn; u++)
+        for (uint32_t v = 0; v < n; v++) out[(size_t)u * n + v] = (uint8_t)oc_get(&e, p->enc_pos[u], v);
+    free(snap); free(alive); free(e.bits); free(p);
+    return 0;
 }
 
 /* ---------------------------------------------------------- DSM text file */

@@ -249,34 +249,39 @@ class DsmFile:
         return self.add[s:e], self.src[s:e], self.dst[s:e], self.clab[s:e]
 
 
-def run_stream_threaded(graph, queries, batches, relabel=True, threads=None, parts=8):
+def run_stream_threaded(graph, queries, batches, relabel=True, threads=None, parts=8, log=None):
     """run_stream's counts (no tuples) with the counting spread over host
     threads: per phase, the candidate sets of every query, then (query, 1/parts
     of the records) tasks. A count is a sum over records (main.cu:268-329), so
-    the split does not change it. The oracle's C calls release the GIL."""
-    from concurrent.futures import ThreadPoolExecutor
+    the split does not change it. The oracle's C calls release the GIL. log
+    (optional) is called with a line as tasks finish."""
+    import time
+    from concurrent.futures import ThreadPoolExecutor, as_completed
     if threads is None:
         threads = max(1, min(16, os.cpu_count() or 1))
     out = []
+    t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        def phase(s_, d_):
+        def phase(name, s_, d_):
             cands = list(ex.map(graph.candidates, queries))
             chunks = [c for c in np.array_split(np.arange(len(s_)), parts) if len(c)]
             tasks = [(qi, c) for qi in range(len(queries)) for c in chunks]
-            sums = list(ex.map(lambda t: int(graph.count_records(queries[t[0]], s_[t[1]], d_[t[1]],
-                                                                 cands[t[0]]).sum()), tasks))
+            futs = {ex.submit(lambda t: int(graph.count_records(queries[t[0]], s_[t[1]], d_[t[1]],
+                                                                cands[t[0]]).sum()), t): t for t in tasks}
             tot = [0] * len(queries)
-            for (qi, _), v in zip(tasks, sums):
-                tot[qi] += v
+            for i, f in enumerate(as_completed(futs)):
+                tot[futs[f][0]] += f.result()
+                if log and (i + 1) % max(1, len(tasks) // 8) == 0:
+                    log("  oracle %s: %d/%d tasks, %.1f s" % (name, i + 1, len(tasks), time.perf_counter() - t0))
             return tot
 
         for add, src, dst, lab in batches:
             add = np.asarray(add, np.uint8)
             src, dst = _u32(src), _u32(dst)
             dels = add == 0
-            res = {"neg": phase(src[dels], dst[dels])}
+            res = {"neg": phase("negatives", src[dels], dst[dels])}
             graph.apply(add, src, dst, lab, relabel)
-            res["pos"] = phase(src[~dels], dst[~dels])
+            res["pos"] = phase("positives", src[~dels], dst[~dels])
             out.append(res)
     return out
 

@@ -130,3 +130,14 @@ def test_cli_usage_error():
     exe = os.path.join(ROOT, "dsm_amd", "bin", "CDSM")
     r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     assert r.returncode == 1 and "usage" in r.stderr
+
+
+def test_intersect_rejects_unsorted_input():
+    # validated on the host before any device call (no GPU needed)
+    from dsm_amd import engine as E
+    import pytest as _pt
+    for a, b in (([3, 2], [1]), ([1, 1], [1]), ([1], [0xFFFFFFFF])):
+        with _pt.raises(E.DsmError) as ei:
+            E.intersect_sorted(a, b)
+        assert ei.value.code == E.DSM_EINVAL
+    assert E.intersect_sorted([], [1, 2]).size == 0

@@ -37,11 +37,12 @@ def qtuples(w):
     return [(q.n, q.labels, q.eu, q.ev) for q in w.queries]
 
 
-def oracle_counts(w, batches):
+def oracle_counts(w, batches, parts=8):
     t = time.perf_counter()
     g = O.Graph(w.n, w.labels, w.eu, w.ev)
+    say("  oracle graph: %.1f s" % (time.perf_counter() - t))
     oq = [O.Query(q.n, q.labels, q.eu, q.ev) for q in w.queries]
-    out = O.run_stream_threaded(g, oq, batches)
+    out = O.run_stream_threaded(g, oq, batches, parts=parts, log=say)
     say("  oracle: %d batches, %d records in %.1f s" % (len(batches), sum(len(b[1]) for b in batches),
                                                        time.perf_counter() - t))
     return out
@@ -140,7 +141,7 @@ def test_rmat16m_reduced_and_full_batch():
     b1 = w.batch(1)
     assert len(b1[1]) == 32768
     batches = [b0, b1]
-    want = oracle_counts(w, batches)
+    want = oracle_counts(w, batches, parts=64)
     g = (w.n, w.labels, w.eu, w.ev)
     eng = E.Engine(g, qtuples(w))
     try:
