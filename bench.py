@@ -59,6 +59,26 @@ GRAPH_TEXT = {"rmat1m_q64": "R-MAT 1M V / 10M E, 16 Zipf(1.0) labels", "rmat1m":
               "rmat16m": "R-MAT 16M V / 256M E, 32 Zipf(0.8) labels", "small": "uniform 1k V / 8k E, 4 labels"}
 
 
+QUERY_RULE = {
+    "rmat1m_q64": "requested as 64 shapes of 4-12 vertices (trees, cycles, 4-cliques, chorded cycles); each drawn by "
+                  "randomized walk with labels copied from the graph and redrawn while its pilot (backtracking-tree "
+                  "nodes over the first 12 batches) exceeds 2e7; rarer labels every 8 redraws, one vertex fewer after "
+                  "160 draws; every 8-12-vertex draw exceeded the budget (dsm_amd/queries/rmat1m_q64.json "
+                  "rejected_sizes; rmat1m_q64_large_pilots.json: pilots above 1.7e9 nodes at 8-12 vertices)",
+    "rmat1m": "one 5-cycle with a chord plus a pendant (6 vertices), randomized walk, pilot budget 5e7",
+    "rmat16m": "4-clique, 5-cycle with two chords and a tree requested at 7 vertices; redrawn while the pilot over "
+               "the first 2 batches exceeds 2e8; 7- and 6-vertex trees never fit, the committed tree has 5 "
+               "(dsm_amd/queries/rmat16m.json rejected_sizes)",
+}
+
+
+def query_histogram(w):
+    h = {}
+    for q in w.queries:
+        h[str(q.n)] = h.get(str(q.n), 0) + 1
+    return dict(sorted(h.items(), key=lambda kv: int(kv[0])))
+
+
 def qtuples(w):
     return [(q.n, q.labels, q.eu, q.ev) for q in w.queries]
 
@@ -85,6 +105,7 @@ def _agg_add(agg, st):
     agg["ms_apply"] += st["ms_apply"]
     agg["ms_filter"] += st["ms_filter"]
     agg["ms_seed"] += st["ms_seed"]
+    agg["ms_refine"] += st["ms_refine"]
     agg["seeds"] += st["seeds_pos"] + st["seeds_neg"]
     agg["donated"] += st["tasks_donated"]
 
@@ -107,7 +128,8 @@ def run_batched(w, warmup, steps, engine_kw, autotune=True, pipelined=True):
         log("autotune %.2f s: %s" % (time.perf_counter() - t0, tuned))
     lat, recs = [], 0
     agg = {"ms_dfs": 0.0, "launches": 0, "entries": 0, "loads": 0, "merge_bytes": 0, "busy_ms": 0.0, "waves": 0,
-           "imbalance": 0.0, "head_idle": 0.0, "tail_idle": 0.0, "ms_apply": 0.0, "ms_filter": 0.0, "ms_seed": 0.0, "seeds": 0, "donated": 0}
+           "imbalance": 0.0, "head_idle": 0.0, "tail_idle": 0.0, "ms_apply": 0.0, "ms_filter": 0.0, "ms_seed": 0.0, "seeds": 0, "donated": 0,
+           "ms_refine": 0.0}
     pos_tot = neg_tot = 0
     per_batch = []
     try:
@@ -405,6 +427,12 @@ def main():
     ap.add_argument("--leaf-budget", type=int, default=0, help="0: the engine's default (2048)")
     ap.add_argument("--overlap-apply", type=int, default=1,
                     help="1: pipelined batches update a second copy of the graph during the round")
+    ap.add_argument("--cluster-filter", type=int, default=0,
+                    help="1: the headline engine refines candidates with the cluster encoding")
+    ap.add_argument("--no-cluster-leg", action="store_true",
+                    help="skip the leg that times the same stream with the cluster-encoding refinement")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-query plain leg (rmat1m: one 6-vertex query, one batch per call)")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run one D2D copy of this many bytes")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -424,7 +452,7 @@ def main():
                  "leaf_sets": bool(args.leaf_sets), "donate_every": args.donate_every,
                  "donate_min_n2": args.donate_min_n2, "leaf_segments": bool(args.leaf_segments),
                  "poll_max": args.poll_max, "overlap_apply": bool(args.overlap_apply),
-                 "leaf_budget": args.leaf_budget}
+                 "leaf_budget": args.leaf_budget, "cluster_filter": bool(args.cluster_filter)}
 
     batch = int(w.bstart[1] - w.bstart[0])
     one = None if args.no_one_by_one else run_batched(w, args.warmup, args.steps, engine_kw,
@@ -452,15 +480,31 @@ def main():
     merge_rate = merge_per_launch / (dfs_ms_avg * 1e-3) / 1e9 if dfs_ms_avg > 0 else 0.0
     # HBM bytes per launch measured by rocprofv3 (FETCH_SIZE / WRITE_SIZE runs
     # scaled by a copy of known size), kept in profiles/
-    traffic = None
+    # The profile was taken on this workload's default run (the same command
+    # under rocprofv3, tuned shape passed explicitly); it carries the
+    # algorithmic bytes of its own launches, so traffic / algorithmic bytes is
+    # a ratio of one run, not of two.
+    traffic = prof = None
     tpath = os.path.join(ROOT, "profiles", "k_dfs_%s_traffic.json" % w.name)
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("traffic_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("traffic_bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": "k_dfs", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(dfs_ms_avg, 4),
+                # what limits the kernel, from its counters (profiles/): waves
+                # wait on dependent loads most of the time, far below the
+                # bandwidth peak
+                "limiter": "memory latency (dependent scattered loads)",
                 "merge_equivalent": {"bytes_per_launch": int(merge_per_launch), "GB_per_s": round(merge_rate, 2)}}
+    if prof and prof.get("algorithmic_bytes_per_launch"):
+        roofline["traffic_run"] = {"algorithmic_bytes_per_launch": prof["algorithmic_bytes_per_launch"],
+                                   "avg_launch_ms": prof.get("avg_launch_ms"),
+                                   "traffic_over_algorithmic": round(traffic / prof["algorithmic_bytes_per_launch"],
+                                                                     3),
+                                   "dram_GB_per_s": prof.get("dram_GB_per_s"),
+                                   "profile": os.path.relpath(tpath, ROOT)}
 
     out = {"metric": "edge updates/s + p50/p99 batch latency, served/batched vs plain GPU path",
            "value": round(value, 1), "unit": "edge updates/s", "n_gpus": 1, "steps": args.steps,
@@ -470,6 +514,9 @@ def main():
                                   "launch per graph state)" % (w.name, GRAPH_TEXT.get(w.name, ""), batch,
                                                                 len(w.queries)),
                       "batch": batch, "queries": len(w.queries),
+                      "query_vertices": query_histogram(w),
+                      "query_selection": QUERY_RULE.get(w.name, ""),
+                      "cluster_filter": bool(args.cluster_filter),
                       "mode": "batched, pipelined (a batch's positives share a launch with the next batch's negatives; the next batch is applied to a second copy of the graph meanwhile)",
                       "parallelism": "single GPU", "tuned": bat["tuned"]},
            "p50_batch_ms": round(pct(bat["lat"], 50) * 1e3, 3), "p99_batch_ms": round(pct(bat["lat"], 99) * 1e3, 3),
@@ -491,6 +538,36 @@ def main():
                            "head_idle": round(agg["head_idle"] / nl, 4),
                            "tail_idle": round(agg["tail_idle"] / nl, 4),
                            "waves_per_launch": int(agg["waves"] / nl)}}
+
+    if not args.no_cluster_leg and w.name == "rmat1m_q64":
+        # the same stream with the other candidate filter: counts must agree
+        kw2 = dict(engine_kw, cluster_filter=not args.cluster_filter)
+        cl = run_batched(w, args.warmup, args.steps, kw2, autotune=not args.no_autotune, pipelined=True)
+        for b, ((p1, n1), (p2, n2)) in enumerate(zip(bat["per_batch"], cl["per_batch"])):
+            assert np.array_equal(p1, p2) and np.array_equal(n1, n2), "filters disagree at batch %d" % b
+        a2 = cl["agg"]
+        out["other_filter"] = {"cluster_filter": kw2["cluster_filter"],
+                               "value": round(cl["records"] / sum(cl["lat"]), 1),
+                               "p50_batch_ms": round(pct(cl["lat"], 50) * 1e3, 3),
+                               "p99_batch_ms": round(pct(cl["lat"], 99) * 1e3, 3),
+                               "dfs_ms_per_launch": round(a2["ms_dfs"] / max(a2["launches"], 1), 4),
+                               "refine_ms_per_batch": round(a2["ms_refine"] / len(cl["lat"]), 3),
+                               "dfs_entries_per_launch": int(a2["entries"] / max(a2["launches"], 1))}
+        out["dfs_entries_per_launch"] = int(agg["entries"] / nl)
+        out["refine_ms_per_batch"] = round(agg["ms_refine"] / len(bat["lat"]), 3)
+
+    if not args.no_single and w.name == "rmat1m_q64":
+        # the single-query workload on the same graph and stream, through the
+        # plain drop-in path (one dsm_process_batch per batch, default shapes)
+        w1 = make_workload("rmat1m")
+        sq = run_batched(w1, args.warmup, args.steps, dict(engine_kw, cluster_filter=False), autotune=False,
+                         pipelined=False)
+        out["single_query_plain"] = {
+            "workload": "rmat1m: %s, %d-record batches, one 6-vertex query (5-cycle with a chord plus a pendant), "
+                        "one dsm_process_batch per batch" % (GRAPH_TEXT["rmat1m"], batch),
+            "value": round(sq["records"] / sum(sq["lat"]), 1), "steps": args.steps,
+            "p50_batch_ms": round(pct(sq["lat"], 50) * 1e3, 3), "p99_batch_ms": round(pct(sq["lat"], 99) * 1e3, 3),
+            "matches_per_s": round((sq["pos"] + sq["neg"]) / sum(sq["lat"]), 1)}
 
     if not args.no_plain:
         ps = min(args.plain_steps, args.steps) if args.plain_steps else args.steps
