@@ -3,8 +3,9 @@
 Default workload: the 1M-vertex R-MAT 64-query stream (R-MAT (.57,.19,.19,.05),
 1,000,000 vertices, 10,000,000 undirected edges, 16 Zipf(1.0) labels; 10% of
 the edges held out as inserts plus 10% of the initial edges as deletes,
-interleaved, in batches of 4,096 records; 64 standing queries of 4-12
-vertices). A step is one update batch: negatives in the pre-batch graph,
+interleaved, in batches of 4,096 records; 64 standing queries requested at
+4-12 vertices, kept at 4-7: every larger draw exceeded the enumeration budget,
+see the line's config). A step is one update batch: negatives in the pre-batch graph,
 the in-place update, candidate refresh and positives in the post-batch graph,
 for all 64 queries.
 

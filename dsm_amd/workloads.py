@@ -14,12 +14,16 @@ Shapes follow the workload definitions of the reference's benchmark plan:
           undirected edges, 16 Zipf(1.0) labels; 10% of edges held out as
           inserts plus 10% of the initial edges as deletes, interleaved, in
           batches of 4,096; one 6-vertex query (5-cycle plus chord).
-  rmat1m_q64  the same graph and stream with 64 standing queries of 4-12
-          vertices (trees, cycles, 4-cliques, chorded cycles).
+  rmat1m_q64  the same graph and stream with 64 standing queries requested
+          as 4-12-vertex trees, cycles, 4-cliques and chorded cycles; the
+          committed set holds 4-7 vertices (21 of 4, 10 of 5, 32 of 6, 1 of 7):
+          every 8-12-vertex draw exceeded the enumeration budget (see
+          draw_queries and queries/rmat1m_q64.json, rejected_sizes).
   rmat16m R-MAT (.57,.19,.19,.05), 16,000,000 V, 256,000,000 undirected
-          edges, 32 Zipf(0.8) labels; the same stream construction in batches
-          of 32,768; dense queries: 4-clique, 5-cycle with two chords,
-          7-vertex tree.
+          edges (the parallel generator), 32 Zipf(0.8) labels; the same stream
+          construction in batches of 32,768; dense queries: 4-clique, 5-cycle
+          with two chords, and a tree requested at 7 vertices that fits the
+          budget only at 5 (queries/rmat16m.json, rejected_sizes).
 
 Label field of an insert record: the ORIGINAL label of its source vertex. Under
 the reference update rule (cpuGraph.cpp:564) an insert relabels both endpoints
@@ -278,27 +282,37 @@ MIN_SIZE = {"tree": 3, "cycle": 3, "chord_cycle": 4, "clique4": 4, "cycle5_chord
             "cycle5_2chords": 5}
 
 
-def _pick_query(n, off, nbr, labels, nl, kind, k, seed0, rs, rd, budget, max_tries, walk=(256, 1024, 5000, 50)):
+def _pick_query(n, off, nbr, labels, nl, kind, k, seed0, rs, rd, budget, max_tries, walk=(256, 1024, 5000, 50),
+                rejected=None):
     """Draws until the pilot fits the budget; every 8 redraws the walk moves
     to rarer labels; after max_tries draws the shape shrinks by one vertex.
-    walk = (max start degree, max walk degree, walk budget, attempts)."""
+    walk = (max start degree, max walk degree, walk budget, attempts).
+    rejected (a list) receives one summary per size given up: draws made,
+    draws whose walk found no embedding, and the sorted pilots of the others
+    (budget + 1 = over the budget)."""
     while k >= MIN_SIZE.get(kind, 3):
+        pilots, missed = [], 0
         for t in range(max_tries):
             q = extract_query(n, off, nbr, labels, kind, k, seed0 + k * 4096 + t, max_start_deg=walk[0],
                               max_walk_deg=walk[1], budget=walk[2], attempts=walk[3],
                               min_label=min(nl - 1, t // 8))
             if q is None:
+                missed += 1
                 continue
             q.cost = pilot_nodes(n, off, nbr, labels, q, rs, rd, budget)
             if q.cost <= budget:
                 q.draw = t
                 return q
+            pilots.append(q.cost)
+        if rejected is not None:
+            rejected.append({"shape": kind, "vertices": k, "draws": max_tries, "no_embedding": missed,
+                             "over_budget": len(pilots), "min_pilot": min(pilots) if pilots else None})
         k -= 1
     raise RuntimeError("no query of shape %s within the budget" % kind)
 
 
 def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=160, threads=8, log=None, walk=None,
-                 fallback=True):
+                 fallback=True, rejected=None):
     """Draws one query per (shape, size) spec by randomized walk on the initial
     graph of w. A draw whose backtracking tree over the records of the first
     `window` batches (pilot on the initial graph, pilot_nodes) has more than
@@ -315,6 +329,8 @@ def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=160, threads=8
     order = np.argsort(-(deg[rs] + deg[rd]), kind="stable")
     rs, rd = np.ascontiguousarray(rs[order]), np.ascontiguousarray(rd[order])
 
+    rej = [[] for _ in specs]
+
     def one(i):
         kind, k = specs[i]
         q = None
@@ -323,7 +339,7 @@ def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=160, threads=8
         for alt in ((kind, "cycle", "tree") if fallback else (kind,)):
             try:
                 q = _pick_query(w.n, off, nbr, w.labels, nl, alt, k, seed * 1000 + i * 1024 + 30, rs, rd,
-                                budget, max_tries, **({"walk": walk} if walk else {}))
+                                budget, max_tries, rejected=rej[i], **({"walk": walk} if walk else {}))
                 break
             except RuntimeError:
                 continue
@@ -335,7 +351,13 @@ def draw_queries(w, specs, seed, nl, budget, window=12, max_tries=160, threads=8
         return q
 
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        return list(ex.map(one, range(len(specs))))
+        out = list(ex.map(one, range(len(specs))))
+    if rejected is not None:
+        for i, r in enumerate(rej):
+            for x in r:
+                x["spec"] = i
+                rejected.append(x)
+    return out
 
 
 QUERY_DIR = os.path.join(_HERE, "queries")
@@ -369,7 +391,8 @@ def rmat(n=1000000, m=10000000, nl=16, zipf=1.0, batch=4096, seed=2, frac=0.10, 
 
 
 def q64_specs(seed=7):
-    """64 query shapes of 4-12 vertices: trees, cycles, 4-cliques, chorded cycles."""
+    """64 requested query shapes of 4-12 vertices: trees, cycles, 4-cliques,
+    chorded cycles (draw_queries shrinks those with no draw within the budget)."""
     p = permutation(4096, seed)
     kinds = ["tree", "cycle", "clique4", "chord_cycle"]
     specs = []
