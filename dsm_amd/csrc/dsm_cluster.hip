@@ -16,8 +16,10 @@
 //                    pass's core condition for all 64 vertices with one load
 //                    per condition row, writes the core and alive words, then
 //                    the wave walks every candidate's neighbours (64 at a
-//                    time) and sets or clears the targets' bits; candidates of
-//                    more than kClusterChunk neighbours are queued in chunks
+//                    time; of a list longer than 64 entries only the label
+//                    buckets of the pass's targets, from the label-sorted
+//                    copy) and sets or clears the targets' bits; buckets of
+//                    more than kClusterChunk entries are queued in chunks
 //   k_cluster_heavy  waves claim those chunks from a counter
 //
 // A pass only reads rows of earlier clusters and only writes its own
@@ -47,25 +49,68 @@ __device__ __forceinline__ bool has(const uint64_t *bm, uint32_t words, uint32_t
   return (ld_relaxed(&bm[(size_t)row * words + (v >> 6)]) >> (v & 63)) & 1ull;
 }
 
-// The targets of one pass for neighbour w of a candidate (lane-parallel: every
-// lane its own w). Encode, merge and new combines set a target's bit when its
+// Target t of a pass at neighbour w of a candidate (lane-parallel: every lane
+// its own w). Encode, merge and new combines set the target's bit when its
 // condition rows all hold w; combines into an existing cluster clear it when
 // they do not.
+__device__ __forceinline__ void update_target(const ClusterArgs &a, const ClusterOp &op, uint32_t kind, uint32_t t,
+                                              uint32_t w) {
+  bool cond = true;
+  const uint32_t o = op.toff[t], c = op.tcnt[t];
+  for (uint32_t k = 0; k < c && cond; k++) cond = has(a.bm, a.words, a.crow[o + k], w);
+  unsigned long long *word = (unsigned long long *)&a.bm[(size_t)(op.core + 1 + t) * a.words + (w >> 6)];
+  const unsigned long long bit = 1ull << (w & 63);
+  if (kind == kOpCombineKeep) {
+    if (!cond && (ld_relaxed(word) & bit)) atomicAnd(word, ~bit);
+  } else if (cond && !(ld_relaxed(word) & bit)) {
+    atomicOr(word, bit);
+  }
+}
+
+// A neighbour of unknown label: the targets of its label (every condition
+// row implies the target's label, and a combine into an existing cluster
+// holds no bit of another label to clear).
 __device__ __forceinline__ void visit_neighbour(const ClusterArgs &a, const ClusterOp &op, uint32_t kind,
                                                 uint32_t w) {
   const uint8_t lw = a.g.lab[w];
-  for (uint32_t t = 0; t < op.ntarget; t++) {
-    if (lw != op.tlab[t]) continue;  // every condition row implies the target's label
-    bool cond = true;
-    const uint32_t o = op.toff[t], c = op.tcnt[t];
-    for (uint32_t k = 0; k < c && cond; k++) cond = has(a.bm, a.words, a.crow[o + k], w);
-    unsigned long long *word = (unsigned long long *)&a.bm[(size_t)(op.core + 1 + t) * a.words + (w >> 6)];
-    const unsigned long long bit = 1ull << (w & 63);
-    if (kind == kOpCombineKeep) {
-      if (!cond && (ld_relaxed(word) & bit)) atomicAnd(word, ~bit);
-    } else if (cond && !(ld_relaxed(word) & bit)) {
-      atomicOr(word, bit);
+  for (uint32_t t = 0; t < op.ntarget; t++)
+    if (lw == op.tlab[t]) update_target(a, op, kind, t, w);
+}
+
+// A neighbour from label bucket li of a label-sorted list.
+__device__ __forceinline__ void visit_bucket_entry(const ClusterArgs &a, const ClusterOp &op, uint32_t kind,
+                                                   uint32_t li, uint32_t w) {
+  for (uint32_t t = 0; t < op.ntarget; t++)
+    if (op.tlix[t] == li) update_target(a, op, kind, t, w);
+}
+
+// The neighbours of candidate v that the targets of pass j can use. A list of
+// more than kLabelSorted entries is read only in the label buckets of the
+// targets (its label-sorted copy); a bucket of more than kClusterChunk entries
+// is queued in chunks for k_cluster_heavy.
+__device__ __forceinline__ void walk_candidate(const ClusterArgs &a, const ClusterOp &op, uint32_t kind, uint32_t j,
+                                               uint32_t v) {
+  const uint32_t lane = lane_id();
+  const uint32_t st = a.g.start[v], dg = a.g.deg[v];
+  if (dg <= kLabelSorted) {
+    for (uint32_t i = lane; i < dg; i += 64) visit_neighbour(a, op, kind, a.g.pool[st + i]);
+    return;
+  }
+  const uint32_t *row = a.g.loff + (size_t)v * (a.g.nbuckets + 1);
+  for (uint32_t k = 0; k < op.nlix; k++) {
+    const uint32_t li = op.lix[k];
+    const uint32_t b0 = row[li], b1 = row[li + 1];
+    if (b1 - b0 > kClusterChunk) {
+      const uint32_t nch = (b1 - b0 + kClusterChunk - 1) / kClusterChunk;
+      uint32_t first = 0;
+      if (lane == 0) first = atomicAdd(&a.ctl[0], nch);
+      first = bcast(first);
+      for (uint32_t c = lane; c < nch; c += 64)
+        if (first + c < a.heavy_cap) a.heavy[first + c] = ClusterHeavy{j, v, b0 + c * kClusterChunk, li};
+      continue;
     }
+    const uint32_t *L = a.g.lpool + st;
+    for (uint32_t i = b0 + lane; i < b1; i += 64) visit_bucket_entry(a, op, kind, li, L[i]);
   }
 }
 
@@ -120,18 +165,7 @@ __global__ __launch_bounds__(kClusterWaves * 64) void k_cluster_pass(ClusterArgs
       while (bits) {
         const uint32_t v = base + (uint32_t)__builtin_ctzll(bits);
         bits &= bits - 1;
-        const uint32_t st = a.g.start[v], dg = a.g.deg[v];
-        if (dg > kClusterChunk) {
-          // queued in chunks for k_cluster_heavy
-          const uint32_t nch = (dg + kClusterChunk - 1) / kClusterChunk;
-          uint32_t first = 0;
-          if (lane == 0) first = atomicAdd(&a.ctl[0], nch);
-          first = bcast(first);
-          for (uint32_t c = lane; c < nch; c += 64)
-            if (first + c < a.heavy_cap) a.heavy[first + c] = ClusterHeavy{j, v, c * kClusterChunk, 0};
-          continue;
-        }
-        for (uint32_t i = lane; i < dg; i += 64) visit_neighbour(a, op, kind, a.g.pool[st + i]);
+        walk_candidate(a, op, kind, j, v);
       }
     }
   }
@@ -146,12 +180,14 @@ __global__ __launch_bounds__(kClusterWaves * 64) void k_cluster_heavy(ClusterArg
     const uint32_t total = bcast(ld_relaxed(&a.ctl[0]));
     if (e >= total || e >= a.heavy_cap) break;
     const ClusterHeavy h = a.heavy[e];
-    const uint32_t j = bcast(h.op), v = bcast(h.v), lo = bcast(h.lo);
+    const uint32_t j = bcast(h.op), v = bcast(h.v), lo = bcast(h.lo), li = bcast(h.li);
     const ClusterOp &op = a.ops[j];
     const uint32_t kind = bcast(op.kind);
-    const uint32_t st = a.g.start[v], dg = a.g.deg[v];
-    const uint32_t hi = min(dg, lo + kClusterChunk);
-    for (uint32_t i = lo + lane; i < hi; i += 64) visit_neighbour(a, op, kind, a.g.pool[st + i]);
+    const uint32_t st = a.g.start[v];
+    const uint32_t b1 = a.g.loff[(size_t)v * (a.g.nbuckets + 1) + li + 1];
+    const uint32_t hi = min(b1, lo + kClusterChunk);
+    const uint32_t *L = a.g.lpool + st;
+    for (uint32_t i = lo + lane; i < hi; i += 64) visit_bucket_entry(a, op, kind, li, L[i]);
   }
 }
 

@@ -331,7 +331,13 @@ int build_cluster_ops(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq) {
       op.core = row(q, c.pos);
       op.ntarget = (uint32_t)c.members.size() - 1;
       op.check_off = (uint32_t)crow.size();
-      for (uint32_t t = 0; t < op.ntarget; t++) op.tlab[t] = (uint8_t)Q.lab[c.members[t + 1]];
+      for (uint32_t t = 0; t < op.ntarget; t++) {
+        op.tlab[t] = (uint8_t)Q.lab[c.members[t + 1]];
+        op.tlix[t] = e->lmap[Q.lab[c.members[t + 1]]];
+        bool seen = false;
+        for (uint32_t k = 0; k < op.nlix; k++) seen = seen || op.lix[k] == op.tlix[t];
+        if (!seen) op.lix[op.nlix++] = op.tlix[t];
+      }
       if (pass.kind == ClusterPlan::kEncode) {
         // encodeKernel: a neighbour of a core candidate is a target's candidate
         op.kind = kOpEncode;

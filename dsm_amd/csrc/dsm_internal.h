@@ -12,7 +12,11 @@ constexpr int kWave = 64;       // CDNA wavefront
 constexpr int kMaxRel = 32;     // labels that can appear in an NLF requirement
 constexpr int kMaxVertexChanges = 1024;  // per vertex per apply round (host splits beyond)
 constexpr uint8_t kNoLabel = 255;  // device label of vertices whose label no query uses
-constexpr uint32_t kLabelSorted = 64;  // vertices of higher degree keep a label-sorted list copy
+#ifndef DSM_LABEL_SORTED
+#define DSM_LABEL_SORTED 64
+#endif
+constexpr uint32_t kLabelSorted = DSM_LABEL_SORTED;  // vertices of higher degree keep a label-sorted list copy
+static_assert(kLabelSorted <= 64, "the heavy refilter reads the label-sorted copies of lists longer than 64");
 
 // Matching plan of one (query, ordered query edge (u0,u1)) pair: positions 0
 // and 1 hold the seeded edge, later positions follow a greedy order in which
@@ -264,10 +268,16 @@ struct alignas(16) ClusterOp {
   uint16_t toff[kMaxQ]; // target t: the AND of rows crow[toff[t] .. +tcnt[t]) at the neighbour
   uint16_t tcnt[kMaxQ];
   uint8_t tlab[kMaxQ];  // device label of target t's query vertex (every condition implies it)
+  uint8_t tlix[kMaxQ];  // its dense label index (the bucket of the label-sorted lists)
+  uint8_t lix[kMaxQ];   // the distinct tlix values, nlix of them
+  uint32_t nlix;
+  uint32_t pad2[3];
 };
 
 struct ClusterHeavy {
-  uint32_t op, v, lo, pad;  // neighbours [lo, lo + kClusterChunk) of candidate v of op
+  // entries [lo, lo + kClusterChunk) of candidate v's label bucket li (in the
+  // label-sorted copy) for pass op
+  uint32_t op, v, lo, li;
 };
 constexpr uint32_t kClusterChunk = 4096;
 
