@@ -38,7 +38,7 @@ EXPORTS = (
     "dsm_file_graph", "dsm_file_query", "dsm_file_num_batches", "dsm_file_batch", "dsm_file_error",
     "dsm_server_create", "dsm_server_submit", "dsm_server_wait", "dsm_server_error", "dsm_server_destroy",
     "dsm_admit", "dsm_estimate_request_bytes", "dsm_autotune", "dsm_process_batches",
-    "dsm_get_refined_candidates", "dsm_cluster_plan", "dsm_intersect_sorted",
+    "dsm_get_refined_candidates", "dsm_cluster_plan", "dsm_intersect_sorted", "dsm_intersect_sorted_noalloc",
 )
 
 
@@ -153,6 +153,8 @@ def lib():
     L.dsm_get_refined_candidates.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
     L.dsm_intersect_sorted.restype = C.c_int
     L.dsm_intersect_sorted.argtypes = [C.c_int, vp, C.c_uint32, vp, C.c_uint32, vp, C.POINTER(C.c_uint32)]
+    L.dsm_intersect_sorted_noalloc.restype = C.c_int
+    L.dsm_intersect_sorted_noalloc.argtypes = [C.c_int, vp, C.c_uint32, vp, C.c_uint32, vp, C.POINTER(C.c_uint32)]
     L.dsm_cluster_plan.restype = C.c_uint32
     L.dsm_cluster_plan.argtypes = [C.c_uint32, C.c_uint32, vp, vp, vp, C.c_uint32]
     L.dsm_get_query_time.restype = C.c_int
@@ -429,14 +431,15 @@ def cluster_plan_words(n, edges):
     return buf[:w].copy() if w else None
 
 
-def intersect_sorted(a, b, device=0):
-    """a ∩ b on the device (dsm_intersect_sorted); a and b strictly ascending."""
+def intersect_sorted(a, b, device=0, noalloc=False):
+    """a ∩ b on the device (dsm_intersect_sorted, or with noalloc
+    dsm_intersect_sorted_noalloc); a and b strictly ascending."""
     L = lib()
     a, b = _u32(a), _u32(b)
     out = np.zeros(max(1, min(len(a), len(b))), np.uint32)
     n = C.c_uint32()
-    rc = L.dsm_intersect_sorted(device, a.ctypes.data, len(a), b.ctypes.data, len(b), out.ctypes.data,
-                                C.byref(n))
+    fn = L.dsm_intersect_sorted_noalloc if noalloc else L.dsm_intersect_sorted
+    rc = fn(device, a.ctypes.data, len(a), b.ctypes.data, len(b), out.ctypes.data, C.byref(n))
     if rc != DSM_OK:
         raise DsmError(rc, L.dsm_last_error(None).decode())
     return out[: n.value].copy()

@@ -281,6 +281,16 @@ int dsm_get_query_time(dsm_engine *e, double *ms, uint32_t num_queries);
 int dsm_intersect_sorted(int device, const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out,
                          uint32_t *num_out);
 
+/* dsm_intersect_sorted without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Its device
+ * memory is a fixed array of 4,194,304 words per GPU, held until the process
+ * ends; a call needs na + nb + 66 words per 64 entries of a (rounded up) +
+ * min(na, nb) + 1 of them, and a larger one fails with DSM_ELIMIT before any
+ * device work. Calls that overlap on one GPU share that array and race. */
+int dsm_intersect_sorted_noalloc(int device, const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb,
+                                 uint32_t *out, uint32_t *num_out);
+
 /* Number of vertices and queries of the engine. */
 uint32_t dsm_num_vertices(const dsm_engine *e);
 uint32_t dsm_num_queries(const dsm_engine *e);
