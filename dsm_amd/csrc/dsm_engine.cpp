@@ -774,6 +774,8 @@ DfsArgs dfs_args(dsm_engine *e) {
   da.lane_work = e->lane_work;
   da.leaf_budget = e->leaf_budget;
   da.bit_probe_last = e->bit_probe_last ? 1u : 0u;
+  da.fshift = 0;
+  while ((e->n > 0 ? (e->n - 1) >> da.fshift : 0u) >= 512u) da.fshift++;
   da.donate_every = e->donate_every;
   da.donate_min_n2 = e->donate_min_n2;
   da.leaf_segments = e->leaf_segments ? 1u : 0u;
@@ -1033,12 +1035,22 @@ int apply_round(dsm_engine *e, const HostDelta &d, const std::vector<uint32_t> &
   b.gdcum = e->bbuf.p + o_gd;
   b.big_merge = e->bbuf.p + o_bm;
   b.big_commit = e->bbuf.p + o_bc;
-  // scratch must hold every touched list once plus the inserts
+  // scratch must hold every touched list once plus the inserts: at most the
+  // pool's entries plus this round's inserts (the stream is idle here, so a
+  // larger scratch replaces the old one without copying)
+  const unsigned long long need = e->pool_cap + ins.size() + (1ull << 20);
+  if (need > e->scratch_cap) {
+    HIPCHK(e->scratch.alloc(need + need / 4));
+    e->scratch_cap = need + need / 4;
+  }
   HIPCHK(launch_scratch_reset(e->actl.p, e->stream));
   HIPCHK(launch_apply_merge(e->dev_graph(), b, e->scratch_cap, e->scratch.p, e->actl.p, e->stream));
   HIPCHK(launch_apply_commit(e->dev_graph(), b, e->scratch.p, e->pool_cap, e->actl.p, e->dirty(), e->stream));
   HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
+  if (e->h_actl.p->merge_overflow)
+    return e->fail_msg(DSM_ELIMIT, "update: %u merged lists found no room in the merge scratch",
+                       e->h_actl.p->merge_overflow);
   for (int attempt = 0; e->h_actl.p->overflow && attempt < 4; attempt++) {
     int rc = grow_pool(e, e->pool_cap / 2 + (1ull << 22));
     if (rc) return rc;
@@ -1069,7 +1081,8 @@ int grow_pool(dsm_engine *e, unsigned long long extra) {
   DevBuf<uint32_t> np, nst;
   HIPCHK(np.alloc(ncap));
   HIPCHK(nst.alloc(n));
-  HIPCHK(hipMemcpy(nst.p, st_new.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  // on the engine's stream, ahead of the kernels that read the new starts
+  HIPCHK(hipMemcpyAsync(nst.p, st_new.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream));
   HIPCHK(launch_pool_move(e->pool.p, np.p, e->start.p, nst.p, e->deg.p, n, e->stream));
   HIPCHK(hipMemcpyAsync(e->cap.p, cp_new.data(), (size_t)n * 4, hipMemcpyHostToDevice, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -1098,7 +1111,10 @@ int grow_pool(dsm_engine *e, unsigned long long extra) {
   ApplyCtl a = *e->h_actl.p;
   a.pool_used = total;
   a.overflow = 0;
-  HIPCHK(hipMemcpy(e->actl.p, &a, sizeof a, hipMemcpyHostToDevice));
+  // ordered before the retried commits on the engine's stream; `a` must stay
+  // alive until the copy is done
+  HIPCHK(hipMemcpyAsync(e->actl.p, &a, sizeof a, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
   *e->h_actl.p = a;
   return DSM_OK;
 }
@@ -1402,7 +1418,9 @@ int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats, bool keep = tru
     if (int r = copy_graph(e, e->shadow)) return r;
     if (int r = apply_delta_and_refresh(e, d)) return r;
     if (stats) HIPCHK(hipEventRecord(e->ev[2], e->stream));
+    HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    if (stats) e->stats.filter_entries += e->h_actl.p->filter_entries;
     if (stats) {
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
@@ -1551,6 +1569,7 @@ int dsm_apply_batch(dsm_engine *e, const dsm_change *changes, uint32_t k) {
     take_refine_time(e);
   }
   e->stats.ms_filter = 0;
+  if (!applied) e->stats.filter_entries = e->h_actl.p->filter_entries;
   e->stats.relabeled_vertices = e->h_actl.p->relabeled;
   e->stats.dirty_groups = e->h_actl.p->ndirty;
   e->stats.pool_entries_used = e->h_actl.p->pool_used;

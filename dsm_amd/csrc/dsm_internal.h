@@ -92,8 +92,11 @@ struct alignas(64) ApplyCtl {
   uint32_t nrelabel_big;            // relabelled vertices whose neighbours the whole grid marks
   uint32_t nmerge_big;              // touched vertices whose merge the whole grid copies
   uint32_t ncommit_big;             // touched vertices whose commit the whole grid copies
-  uint32_t pad[4];
+  uint32_t merge_overflow;          // merges that found no room in the scratch (must stay 0)
+  uint32_t pad;
+  unsigned long long filter_entries;  // list entries of the vertices re-filtered (work measure)
 };
+static_assert(sizeof(ApplyCtl) == 64, "apply status layout");
 
 struct DevGraph {
   uint32_t n;
@@ -165,6 +168,7 @@ struct DfsArgs {
   uint32_t poll_max;             // longest sleep between polls of a waiting wave (s_sleep 8 units)
   uint32_t leaf_sets;            // fused leaf level on/off
   uint32_t bit_probe_last;       // candidate bit probed after the membership tests
+  uint32_t fshift;               // ids >> fshift < 512: the leaf set's bucket of an id
   uint32_t nq;                   // queries (count layout)
   const uint32_t *seed_tag;      // per seed: count slot << 1 | sign
   unsigned long long watchdog;   // wall-clock ticks an idle wave may wait

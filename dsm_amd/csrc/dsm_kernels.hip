@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
       if (lane == 0) {
         b.ndeg[t] = na;
         b.soff[t] = ~0ull;
-        atomicAdd(&actl->overflow, 1u);
+        atomicAdd(&actl->merge_overflow, 1u);
       }
       continue;
     }
@@ -413,12 +413,14 @@ __global__ __launch_bounds__(256) void k_filter_light(DevGraph g, FilterTables f
   __syncthreads();
   const uint32_t nd = d.actl->ndirty < d.list_cap ? d.actl->ndirty : d.list_cap;
   uint8_t *cnt = s_cnt[threadIdx.x];
+  unsigned long long entries = 0;  // list entries of the vertices this thread re-filters
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += gridDim.x * blockDim.x) {
     const uint32_t v = d.list[i];
     atomicAnd(&d.bits[v >> 5], ~(1u << (v & 31)));
     const uint32_t L = g.lab[v];
     if (L == kNoLabel || ft.sig_lo[L] == ft.sig_hi[L]) continue;
     const uint32_t dv = g.deg[v];
+    entries += dv;
     if (dv > (uint32_t)kLight) {
       const uint32_t p = atomicAdd(&d.actl->nheavy, 1u);
       if (p < d.list_cap) d.heavy[p] = v;
@@ -439,6 +441,8 @@ __global__ __launch_bounds__(256) void k_filter_light(DevGraph g, FilterTables f
     for (uint32_t s = ft.sig_lo[L]; s < ft.sig_hi[L]; s++)
       set_bit(&ft.bm[(size_t)s * ft.words + (v >> 6)], v, sig_pass(ft.sigs[s], dv, c));
   }
+  for (int o = 32; o >= 1; o >>= 1) entries += __shfl_xor(entries, o);
+  if (lane_id() == 0 && entries) atomicAdd(&d.actl->filter_entries, entries);
 }
 
 // Dirty vertices with long lists (every one has a label-sorted copy, rebuilt
@@ -847,6 +851,8 @@ __global__ void k_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tup
     if (actl) {
       actl->scratch_used = 0;
       actl->overflow = 0;
+      actl->merge_overflow = 0;
+      actl->filter_entries = 0;
       actl->relocated = 0;
       actl->ndirty = 0;
       actl->nheavy = 0;
@@ -887,6 +893,12 @@ struct WaveHdr {
   uint32_t fstate, fn;       // leaf set F of the current positions 0..n-3 (see leaf_count)
 };
 constexpr uint32_t kFCap = 2048;  // largest leaf set F kept in LDS
+// F is indexed by id range: bucket b = id >> DfsArgs::fshift (fshift makes the
+// graph's ids fit kFIdx buckets), fidx[b] = first entry of F at or past the
+// bucket. A membership test reads two offsets and binary-searches only the
+// bucket's entries (a few, ids being spread over the range) instead of
+// log2 |F| dependent LDS reads over all of F.
+constexpr uint32_t kFIdx = 512;
 // One level above the leaf, the candidates of a 64-entry window count their
 // leaf images inline until their summed scan or search work passes
 // DfsArgs::leaf_budget entries; the candidate that passes it descends to the
@@ -905,10 +917,12 @@ struct WaveMem {
   uint32_t *cst;    // [levels][64] list start of each candidate
   uint32_t *cdg;    // [levels][64] degree of each candidate
   uint32_t *fbuf;   // [kFCap] leaf set F, ascending
+  uint16_t *fidx;   // [kFIdx + 1] bucket offsets into fbuf
 };
 
 __host__ __device__ constexpr uint32_t wave_lds_bytes(uint32_t chunk_levels) {
-  return (uint32_t)((sizeof(WaveHdr) + 15) / 16 * 16) + kWin * 4 + kFCap * 4 + chunk_levels * 64 * 12;
+  return (uint32_t)((sizeof(WaveHdr) + 15) / 16 * 16) + kWin * 4 + kFCap * 4 + chunk_levels * 64 * 12 +
+         ((kFIdx + 1) * 2 + 15) / 16 * 16;
 }
 
 // 4 B per entry of the full lists of the positions in `mask`: what a merge
@@ -1052,11 +1066,31 @@ __device__ void compute_leaf_set(const WaveMem &M, const DfsArgs &a, uint32_t le
     }
   }
   wave_sync();
+  if (state == kFReady) {
+    // bucket offsets: entry i opens every bucket after its predecessor's up
+    // to its own; buckets past the last entry start at cnt
+    const uint32_t sh = a.fshift;
+    for (uint32_t i = lane; i < cnt; i += 64) {
+      const uint32_t b = M.fbuf[i] >> sh;
+      const uint32_t pb = i ? (M.fbuf[i - 1] >> sh) + 1 : 0u;
+      for (uint32_t k = pb; k <= b; k++) M.fidx[k] = (uint16_t)i;
+    }
+    const uint32_t last = cnt ? (M.fbuf[cnt - 1] >> sh) + 1 : 0u;
+    for (uint32_t k = last + lane; k <= kFIdx; k += 64) M.fidx[k] = (uint16_t)cnt;
+  }
+  wave_sync();
   if (lane == 0) {
     S.fstate = state;
     S.fn = cnt;
   }
   wave_sync();
+}
+
+// y in F (state kFReady), through the bucket index.
+__device__ __forceinline__ bool in_leaf_set(const WaveMem &M, const DfsArgs &a, uint32_t y) {
+  const uint32_t b = y >> a.fshift;
+  const uint32_t lo = M.fidx[b], hi = M.fidx[b + 1];
+  return lds_contains(M.fbuf + lo, hi - lo, y);
 }
 
 // Leaf images of the partial match whose position n-2 holds w (this lane's
@@ -1077,7 +1111,7 @@ struct LeafList {
 __device__ __forceinline__ bool leaf_accept(const WaveHdr &S, const DfsArgs &a, const WaveMem &M, uint32_t leaf,
                                             uint32_t st, uint32_t fn, bool injw, uint32_t w, uint32_t y) {
   if (injw && y == w) return false;
-  if (st == kFReady) return lds_contains(M.fbuf, fn, y);
+  if (st == kFReady) return in_leaf_set(M, a, y);
   if (!bit_of(a.bm, a.words, S.sig[leaf], y)) return false;
   uint32_t m = S.inj[leaf] & ~(1u << (leaf - 1));
   while (m) {
@@ -1099,7 +1133,7 @@ __device__ __forceinline__ uint32_t leaf_count_lane(const WaveMem &M, const DfsA
   defer = false;
   if (!((back >> pv) & 1)) {
     uint32_t c = fn;
-    if (injw && lds_contains(M.fbuf, fn, w)) c--;
+    if (injw && in_leaf_set(M, a, w)) c--;
     return c;
   }
   const uint32_t ws = a.g.start[w], wd = a.g.deg[w];
@@ -1342,6 +1376,7 @@ __global__ __launch_bounds__(WPB * 64) void k_dfs(DfsArgs a, uint32_t chunk_leve
     M.cst = M.cand + chunk_levels * 64;
     M.cdg = M.cst + chunk_levels * 64;
     M.fbuf = M.cdg + chunk_levels * 64;
+    M.fidx = reinterpret_cast<uint16_t *>(M.fbuf + kFCap);
   }
   WaveHdr &S = *M.h;
   if (lane == 0) S.plan = 0xFFFFFFFFu;

@@ -94,7 +94,7 @@ class Stats(C.Structure):
                 ("dfs_entries", C.c_uint64), ("dfs_search_loads", C.c_uint64), ("dfs_launches", C.c_uint32),
                 ("dfs_merge_bytes", C.c_uint64), ("dfs_busy_ms", C.c_double), ("dfs_waves", C.c_uint64),
                 ("dfs_block_imbalance", C.c_double), ("dfs_head_idle", C.c_double), ("dfs_tail_idle", C.c_double),
-                ("ms_refine", C.c_float)]
+                ("ms_refine", C.c_float), ("filter_entries", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

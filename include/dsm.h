@@ -145,6 +145,8 @@ typedef struct {
   double dfs_head_idle;
   double dfs_tail_idle;
   float ms_refine;                 /* device time of the cluster-encoding refinement */
+  uint64_t filter_entries;         /* list entries of the vertices whose candidacy was
+                                      re-evaluated (the refilter's work measure) */
 } dsm_stats;
 
 /* Outcome of dsm_autotune: the launch parameters kept and the DFS time of the

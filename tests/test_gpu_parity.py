@@ -638,3 +638,22 @@ def test_tuple_capacity_truncation():
             assert tuple(row[2:5].tolist()) in real
     finally:
         eng.close()
+
+
+def test_insert_batch_larger_than_merge_scratch():
+    """One batch of 1.5M inserts on a 3,000-vertex graph: an apply round
+    merges up to 1,024 inserts into each of the 3,000 lists, more than the
+    pool's entries plus 2^20, so the merge scratch must grow before the merge
+    (before the fix such merges were dropped silently). Graph and counts
+    against the oracle; the query's labels keep the count small."""
+    say("case: 1.5M inserts into 3,000 vertices (merge scratch growth)")
+    rng = np.random.default_rng(4242)
+    n = 3000
+    labels, eu, ev = random_graph(n, 6000, 2, rng)
+    labels[:5] = 2  # five vertices carry the query's label
+    queries = [qtuple("triangle", [2, 2, 2])]
+    k = 1_500_000
+    src = rng.integers(0, n, k).astype(np.uint32)
+    dst = rng.integers(0, n, k).astype(np.uint32)
+    big = (np.ones(k, np.uint8), src, dst, np.zeros(k, np.uint32))
+    compare_stream(n, labels, eu, ev, queries, [big], relabel=False, check_graph=True)
