@@ -109,6 +109,9 @@ def _agg_add(agg, st):
     agg["ms_refine"] += st["ms_refine"]
     agg["seeds"] += st["seeds_pos"] + st["seeds_neg"]
     agg["donated"] += st["tasks_donated"]
+    agg["filter_entries"] += st["filter_entries"]
+    agg["dirty"] += st["dirty_groups"]
+    agg["filter_bits"] += st["filter_bits"]
 
 
 def run_batched(w, warmup, steps, engine_kw, autotune=True, pipelined=True):
@@ -131,7 +134,7 @@ def run_batched(w, warmup, steps, engine_kw, autotune=True, pipelined=True):
     lat, recs = [], 0
     agg = {"ms_dfs": 0.0, "launches": 0, "entries": 0, "loads": 0, "merge_bytes": 0, "busy_ms": 0.0, "waves": 0,
            "imbalance": 0.0, "head_idle": 0.0, "tail_idle": 0.0, "ms_apply": 0.0, "ms_filter": 0.0, "ms_seed": 0.0, "seeds": 0, "donated": 0,
-           "ms_refine": 0.0}
+           "ms_refine": 0.0, "filter_entries": 0, "dirty": 0, "filter_bits": 0}
     pos_tot = neg_tot = 0
     per_batch = []
     try:
@@ -500,6 +503,23 @@ def main():
                 # bandwidth peak
                 "limiter": "memory latency (dependent scattered loads)",
                 "merge_equivalent": {"bytes_per_launch": int(merge_per_launch), "GB_per_s": round(merge_rate, 2)}}
+    # The whole path per batch, in the survey's terms: B = B_upd + B_filt +
+    # B_enum with B_upd = 2 x k x (4 B adjacency entry + 8 B delta record) and
+    # B_filt = per re-filtered vertex 4 B offset + deg x (4 B id + 1 B label),
+    # + 1/8 B per candidate bit written. B_enum, the merge-equivalent bytes, is
+    # a work measure this search does not read (it exceeds HBM's peak when
+    # divided by the batch time); the search's own bytes stand in for it in B.
+    nbt = max(len(bat["lat"]), 1)
+    b_upd = 2.0 * bat["records"] * 12 / nbt
+    b_filt = (agg["dirty"] * 4 + agg["filter_entries"] * 5 + agg["filter_bits"] / 8.0) / nbt
+    b_search = (agg["entries"] * 12 + agg["loads"] * 4) / nbt
+    b_total = b_upd + b_filt + b_search
+    t_batch = total_s / nbt
+    roofline["per_batch"] = {"B_upd": int(b_upd), "B_filt": int(b_filt), "B_search": int(b_search),
+                             "B_enum_merge_equivalent": int((agg["merge_bytes"] + agg["entries"] / 8.0) / nbt),
+                             "B": int(b_total), "t_batch_ms": round(t_batch * 1e3, 3),
+                             "achieved": round(b_total / t_batch / 1e9, 2), "unit": "GB/s",
+                             "frac": round(b_total / t_batch / 1e9 / HBM_PEAK_GBS, 5)}
     if prof and prof.get("algorithmic_bytes_per_launch"):
         roofline["traffic_run"] = {"algorithmic_bytes_per_launch": prof["algorithmic_bytes_per_launch"],
                                    "avg_launch_ms": prof.get("avg_launch_ms"),

@@ -116,6 +116,7 @@ struct dsm_engine {
   DevBuf<uint8_t> lab;
   DevBuf<uint32_t> lpool, loff;  // label-sorted copies of long lists
   DevBuf<uint32_t> huge_list, huge_ctl, huge_hist;  // segmented builds of the longest lists
+  DevBuf<uint32_t> big_list;                         // vertices whose copy a whole block builds
   uint32_t huge_segs = 0;
   uint32_t nbuckets = 1;
   unsigned long long pool_cap = 0;
@@ -262,6 +263,8 @@ struct dsm_engine {
     h.list = huge_list.p;
     h.cap = (uint32_t)(huge_list.n / 2);
     h.ctl = huge_ctl.p;
+    h.big = big_list.p;
+    h.big_cap = (uint32_t)big_list.n;
     h.hist = huge_hist.p;
     h.segs = huge_segs;
     // the merge scratch (at least one entry per pool entry) is idle while the
@@ -591,7 +594,10 @@ int size_huge_sort(dsm_engine *e) {
   }
   if (!e->huge_list.p) {
     HIPCHK(e->huge_list.alloc(2 * 4096));
-    HIPCHK(e->huge_ctl.alloc(2));
+    HIPCHK(e->huge_ctl.alloc(3));
+    // at most 2 m / 4,097 vertices have a degree above 4,096; a longer queue
+    // makes k_lbuild_big look at every listed vertex instead
+    HIPCHK(e->big_list.alloc((size_t)e->n / 64 + 4096));
   }
   return DSM_OK;
 }
@@ -1420,7 +1426,11 @@ int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats, bool keep = tru
     if (stats) HIPCHK(hipEventRecord(e->ev[2], e->stream));
     HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    if (stats) e->stats.filter_entries += e->h_actl.p->filter_entries;
+    if (stats) {
+      e->stats.filter_entries += e->h_actl.p->filter_entries;
+      e->stats.dirty_groups += e->h_actl.p->ndirty;
+      e->stats.filter_bits += e->h_actl.p->filter_bits;
+    }
     if (stats) {
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[2]));
@@ -1569,9 +1579,12 @@ int dsm_apply_batch(dsm_engine *e, const dsm_change *changes, uint32_t k) {
     take_refine_time(e);
   }
   e->stats.ms_filter = 0;
-  if (!applied) e->stats.filter_entries = e->h_actl.p->filter_entries;
+  if (!applied) {
+    e->stats.filter_entries = e->h_actl.p->filter_entries;
+    e->stats.dirty_groups = e->h_actl.p->ndirty;
+    e->stats.filter_bits = e->h_actl.p->filter_bits;
+  }
   e->stats.relabeled_vertices = e->h_actl.p->relabeled;
-  e->stats.dirty_groups = e->h_actl.p->ndirty;
   e->stats.pool_entries_used = e->h_actl.p->pool_used;
   e->stats.pool_entries_capacity = e->pool_cap;
   e->batch_pending = true;

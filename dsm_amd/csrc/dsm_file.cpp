@@ -2,7 +2,8 @@
 //
 //   t <n> <m>
 //   v <id> <label> <deg>        n lines, ids 0..n-1 in order
-//   e <a> <b>                   m lines, each undirected edge once
+//   e <a> <b> [<elabel>]        m lines, each undirected edge once; the
+//                               optional edge label is kept but never matched
 //   -                           data files only; ends the graph
 //   b <bid> <k>                 bid = 0,1,2,... in order
 //   a|d <src> <dst> <label>     k lines
@@ -24,6 +25,8 @@ struct dsm_file {
   uint32_t n = 0;
   uint64_t m = 0;
   std::vector<uint32_t> labels, deg, eu, ev;
+  std::vector<uint32_t> elab;  // edge labels of the e lines (0 where absent)
+  bool has_elab = false;
   std::vector<uint32_t> bstart;
   std::vector<dsm_change> changes;
 };
@@ -37,7 +40,13 @@ struct Reader {
   std::vector<char> buf;
   size_t pos = 0, len = 0;
   bool eof = false;
+  std::string held;  // a token read ahead and given back
+  bool has_held = false;
   explicit Reader(FILE *fp) : f(fp), buf(1 << 20) {}
+  void unget(const std::string &t) {
+    held = t;
+    has_held = true;
+  }
   int getc() {
     if (pos == len) {
       if (eof) return EOF;
@@ -52,6 +61,11 @@ struct Reader {
   }
   // next whitespace-separated token; false at end of input
   bool token(std::string &t) {
+    if (has_held) {
+      t = held;
+      has_held = false;
+      return true;
+    }
     t.clear();
     int c;
     do {
@@ -66,7 +80,10 @@ struct Reader {
   }
   bool u32(uint32_t &out) {
     std::string t;
-    if (!token(t) || t.empty() || t.size() > 10) return false;
+    return token(t) && parse_u32(t, out);
+  }
+  static bool parse_u32(const std::string &t, uint32_t &out) {
+    if (t.empty() || t.size() > 10) return false;
     uint64_t v = 0;
     for (char ch : t) {
       if (ch < '0' || ch > '9') return false;
@@ -131,6 +148,13 @@ int dsm_file_read(const char *path, int is_query, dsm_file **out) {
       if (f->eu.size() >= m32) return fail("more e lines than declared");
       f->eu.push_back(a);
       f->ev.push_back(b);
+      // an edge label, when the next token is a number rather than a line type
+      uint32_t el = 0;
+      if (r.token(t)) {
+        if (Reader::parse_u32(t, el)) f->has_elab = true;
+        else r.unget(t);
+      }
+      f->elab.push_back(el);
     } else if (t == "-") {
       ended = true;
       break;
@@ -193,6 +217,12 @@ int dsm_file_query(const dsm_file *f, dsm_query_desc *q) {
   q->edge_src = f->eu.data();
   q->edge_dst = f->ev.data();
   return DSM_OK;
+}
+
+int dsm_file_edge_labels(const dsm_file *f, const uint32_t **labels) {
+  if (!f || !labels) return DSM_EINVAL;
+  *labels = f->elab.data();
+  return f->has_elab ? 1 : 0;
 }
 
 uint32_t dsm_file_num_batches(const dsm_file *f) { return f ? (uint32_t)f->bstart.size() - 1 : 0; }

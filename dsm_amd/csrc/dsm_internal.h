@@ -93,7 +93,7 @@ struct alignas(64) ApplyCtl {
   uint32_t nmerge_big;              // touched vertices whose merge the whole grid copies
   uint32_t ncommit_big;             // touched vertices whose commit the whole grid copies
   uint32_t merge_overflow;          // merges that found no room in the scratch (must stay 0)
-  uint32_t pad;
+  uint32_t filter_bits;             // candidate bits the re-filter writes (work measure)
   unsigned long long filter_entries;  // list entries of the vertices re-filtered (work measure)
 };
 static_assert(sizeof(ApplyCtl) == 64, "apply status layout");
@@ -326,7 +326,9 @@ constexpr uint32_t kHugeSeg = 65536;
 struct HugeSort {
   uint32_t *list;      // [cap][2]: vertex, first segment slot
   uint32_t cap;
-  uint32_t *ctl;       // [2]: queued vertices, segment slots taken
+  uint32_t *ctl;       // [3]: queued vertices, segment slots taken, vertices on big
+  uint32_t *big;       // [big_cap]: vertices of degree > 4,096 for the block-per-list build
+  uint32_t big_cap;
   uint32_t *hist;      // [segs][kMaxRel + 2]
   uint32_t segs;
   // one byte per pool entry (indexed like the pool): the bucket of each entry,

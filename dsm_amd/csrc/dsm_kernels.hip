@@ -51,6 +51,38 @@ __device__ __forceinline__ void mark_dirty(const DirtyArgs &d, uint32_t x) {
   }
 }
 
+// Only vertices with a label some query uses need their candidacy and their
+// label-sorted copy kept current: no seed, DFS level or refilter reads the
+// copy or the bits of any other vertex. A vertex that takes such a label later
+// is marked by its own relabel, so a stale copy never gets read. (A label read
+// here can race with the vertex's own relabel in the same kernel; either
+// order marks what the relabel needs.)
+__device__ __forceinline__ void mark_dirty_used(const DevGraph &g, const DirtyArgs &d, uint32_t x) {
+  if (g.lab[x] != kNoLabel) mark_dirty(d, x);
+}
+
+// mark_dirty_used for one entry per lane (valid lanes only), called by every
+// lane of the wave alike: the lanes that newly set their vertex's bit take
+// their list slots with one counter add for the wave, so that the marks of a
+// hub's neighbours do not serialise on the counter.
+__device__ __forceinline__ void mark_dirty_wave(const DevGraph &g, const DirtyArgs &d, uint32_t x, bool valid) {
+  bool fresh = false;
+  if (valid && g.lab[x] != kNoLabel) {
+    const uint32_t bit = 1u << (x & 31);
+    if (!(ld_relaxed(&d.bits[x >> 5]) & bit)) fresh = !(atomicOr(&d.bits[x >> 5], bit) & bit);
+  }
+  const uint64_t m = ballot(fresh);
+  if (m == 0) return;
+  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(&d.actl->ndirty, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader);
+  if (fresh) {
+    const uint32_t p = base + mbcnt(m);
+    if (p < d.list_cap) d.list[p] = x;
+  }
+}
+
 // ------------------------------------------------------------------ apply
 
 // One wave per touched vertex: new list = (A \ D) u I written to scratch.
@@ -257,7 +289,7 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_commit(DevGraph g, B
     if (lane == 0) {
       g.deg[x] = nd;
       b.fail[t] = 0;
-      mark_dirty(d, x);
+      mark_dirty_used(g, d, x);
     }
   }
 }
@@ -299,7 +331,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, BatchDev b, FilterT
     if (lane == 0) {
       g.lab[x] = (uint8_t)L;
       atomicAdd(&d.actl->relabeled, 1u);
-      mark_dirty(d, x);
+      if (L != kNoLabel) mark_dirty(d, x);
     }
     if (old != kNoLabel) {
       const uint64_t clr = ~(1ull << (x & 63));
@@ -312,7 +344,10 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, BatchDev b, FilterT
       if (lane == 0) b.rl_big[atomicAdd(&d.actl->nrelabel_big, 1u)] = x;
       continue;
     }
-    for (uint32_t j = lane; j < na; j += 64) mark_dirty(d, A[j]);
+    for (uint32_t j0 = 0; j0 < na; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      mark_dirty_wave(g, d, j < na ? A[j] : 0u, j < na);
+    }
   }
 }
 
@@ -326,7 +361,12 @@ __global__ __launch_bounds__(256) void k_relabel_big(DevGraph g, BatchDev b, Dir
     uint32_t first, stride;
     if (!big_share(r, na, first, stride)) continue;
     const uint32_t *A = g.pool + g.start[x];
-    for (uint32_t j = first; j < na; j += stride) mark_dirty(d, A[j]);
+    // stride is a multiple of 64 and first - lane_id() is the wave's start,
+    // so the lanes of a wave run the loop alike
+    for (uint32_t j0 = first - lane_id(); j0 < na; j0 += stride) {
+      const uint32_t j = j0 + lane_id();
+      mark_dirty_wave(g, d, j < na ? A[j] : 0u, j < na);
+    }
   }
 }
 
@@ -414,6 +454,7 @@ __global__ __launch_bounds__(256) void k_filter_light(DevGraph g, FilterTables f
   const uint32_t nd = d.actl->ndirty < d.list_cap ? d.actl->ndirty : d.list_cap;
   uint8_t *cnt = s_cnt[threadIdx.x];
   unsigned long long entries = 0;  // list entries of the vertices this thread re-filters
+  uint32_t bits = 0;               // candidate bits they receive (one per signature of the label)
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += gridDim.x * blockDim.x) {
     const uint32_t v = d.list[i];
     atomicAnd(&d.bits[v >> 5], ~(1u << (v & 31)));
@@ -421,6 +462,7 @@ __global__ __launch_bounds__(256) void k_filter_light(DevGraph g, FilterTables f
     if (L == kNoLabel || ft.sig_lo[L] == ft.sig_hi[L]) continue;
     const uint32_t dv = g.deg[v];
     entries += dv;
+    bits += ft.sig_hi[L] - ft.sig_lo[L];
     if (dv > (uint32_t)kLight) {
       const uint32_t p = atomicAdd(&d.actl->nheavy, 1u);
       if (p < d.list_cap) d.heavy[p] = v;
@@ -441,8 +483,12 @@ __global__ __launch_bounds__(256) void k_filter_light(DevGraph g, FilterTables f
     for (uint32_t s = ft.sig_lo[L]; s < ft.sig_hi[L]; s++)
       set_bit(&ft.bm[(size_t)s * ft.words + (v >> 6)], v, sig_pass(ft.sigs[s], dv, c));
   }
-  for (int o = 32; o >= 1; o >>= 1) entries += __shfl_xor(entries, o);
+  for (int o = 32; o >= 1; o >>= 1) {
+    entries += __shfl_xor(entries, o);
+    bits += __shfl_xor(bits, o);
+  }
   if (lane_id() == 0 && entries) atomicAdd(&d.actl->filter_entries, entries);
+  if (lane_id() == 0 && bits) atomicAdd(&d.actl->filter_bits, bits);
 }
 
 // Dirty vertices with long lists (every one has a label-sorted copy, rebuilt
@@ -539,11 +585,39 @@ __device__ __forceinline__ void bucket_scatter(const DevGraph &g, const uint8_t 
 constexpr uint32_t kLbuildBlock = 4096;  // lists longer than this are sorted by a whole block
 constexpr uint32_t kLbuildWaves = 16;    // waves of that block
 
-// One wave per vertex of degree in (kLabelSorted, kLbuildBlock]: bucket
-// histogram, offsets, then a stable scatter of the id-sorted list into lpool
-// bucket by bucket. Longer lists are left to k_lbuild_big.
+// The label-sorted copy of v (degree d) by one wave: bucket histogram,
+// offsets, then a stable scatter of the id-sorted list into lpool bucket by
+// bucket.
+__device__ __forceinline__ void build_one(const DevGraph &g, const uint8_t *s_lmap, uint32_t v, uint32_t d,
+                                          uint32_t nb, uint32_t bits, uint32_t *cnt, uint8_t *bucket_bytes) {
+  const uint32_t lane = lane_id();
+  const uint32_t *A = g.pool + g.start[v];
+  uint32_t *row = g.loff + (size_t)v * (nb + 1);
+  if (lane < nb) cnt[lane] = 0;
+  wave_sync();
+  uint8_t *bk = bucket_bytes ? bucket_bytes + g.start[v] : nullptr;
+  bucket_count(g, s_lmap, A, 0, d, bits, cnt, nullptr, bk);
+  if (lane == 0) {
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < nb; k++) {
+      const uint32_t c = cnt[k];
+      cnt[k] = run;
+      row[k] = run;
+      run += c;
+    }
+    row[nb] = run;
+  }
+  wave_sync();
+  bucket_scatter(g, s_lmap, A, 0, d, bits, cnt, g.lpool + g.start[v], bk);
+}
+
+// The listed vertices (or all), a lane each: those of degree in
+// (kLabelSorted, kLbuildBlock] are built here, a wave per vertex in turn; the
+// longer ones are queued on hs.big for k_lbuild_big. Vertices whose label no
+// query uses are skipped (never read, see mark_dirty_used); a full build
+// leaves their groups empty.
 __global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap, const uint32_t *list,
-                                                const uint32_t *count, uint8_t *bucket_bytes) {
+                                                const uint32_t *count, HugeSort hs) {
   __shared__ uint8_t s_lmap[256];
   __shared__ uint32_t s_cnt[4][kMaxRel + 2];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = lmap[i];
@@ -553,28 +627,31 @@ __global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap,
   const uint32_t bits = 32u - (uint32_t)__builtin_clz(nb > 1 ? nb - 1 : 1u);
   const uint32_t total = list ? (*count < g.n ? *count : g.n) : g.n;
   uint32_t *cnt = s_cnt[wv];
-  for (uint32_t t = blockIdx.x * 4 + wv; t < total; t += gridDim.x * 4) {
-    const uint32_t v = list ? list[t] : t;
-    const uint32_t d = g.deg[v];
-    if (d <= kLabelSorted || d > kLbuildBlock) continue;
-    const uint32_t *A = g.pool + g.start[v];
-    uint32_t *row = g.loff + (size_t)v * (nb + 1);
-    if (lane < nb) cnt[lane] = 0;
-    wave_sync();
-    uint8_t *bk = bucket_bytes ? bucket_bytes + g.start[v] : nullptr;
-    bucket_count(g, s_lmap, A, 0, d, bits, cnt, nullptr, bk);
-    if (lane == 0) {
-      uint32_t run = 0;
-      for (uint32_t k = 0; k < nb; k++) {
-        const uint32_t c = cnt[k];
-        cnt[k] = run;
-        row[k] = run;
-        run += c;
-      }
-      row[nb] = run;
+  for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base < total; base += gridDim.x * 256) {
+    const uint32_t t = base + lane;
+    uint32_t vl = 0, dl = 0;
+    bool used = false;
+    if (t < total) {
+      vl = list ? list[t] : t;
+      dl = g.deg[vl];
+      used = g.lab[vl] != kNoLabel;
+      if (!used && !list && dl > kLabelSorted)
+        for (uint32_t k = 0; k <= nb; k++) g.loff[(size_t)vl * (nb + 1) + k] = 0;
     }
-    wave_sync();
-    bucket_scatter(g, s_lmap, A, 0, d, bits, cnt, g.lpool + g.start[v], bk);
+    const uint64_t big = ballot(used && dl > kLbuildBlock);
+    if (big) {
+      const uint32_t leader = (uint32_t)__builtin_ctzll(big);
+      uint32_t at = 0;
+      if (lane == leader) at = atomicAdd(&hs.ctl[2], (uint32_t)__popcll(big));
+      at = __shfl(at, (int)leader) + mbcnt(big);
+      if (((big >> lane) & 1) && at < hs.big_cap) hs.big[at] = vl;
+    }
+    uint64_t mid = ballot(used && dl > kLabelSorted && dl <= kLbuildBlock);
+    while (mid) {
+      const uint32_t k = (uint32_t)__builtin_ctzll(mid);
+      mid &= mid - 1;
+      build_one(g, s_lmap, __shfl(vl, (int)k), __shfl(dl, (int)k), nb, bits, cnt, hs.bucket_bytes);
+    }
   }
 }
 
@@ -595,11 +672,15 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_big(DevGraph g, co
   const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
   const uint32_t nb = g.nbuckets;
   const uint32_t bits = 32u - (uint32_t)__builtin_clz(nb > 1 ? nb - 1 : 1u);
-  const uint32_t total = list ? (*count < g.n ? *count : g.n) : g.n;
+  // the vertices k_lbuild queued; past the queue's capacity, every listed
+  // vertex is looked at again
+  const uint32_t queued = hs.ctl[2];
+  const bool from_queue = queued <= hs.big_cap;
+  const uint32_t total = from_queue ? queued : list ? (*count < g.n ? *count : g.n) : g.n;
   for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-    const uint32_t v = list ? list[t] : t;
+    const uint32_t v = from_queue ? hs.big[t] : list ? list[t] : t;
     const uint32_t d = g.deg[v];
-    if (d <= kLbuildBlock) continue;
+    if (d <= kLbuildBlock || g.lab[v] == kNoLabel) continue;
     if (d > kHugeList) {
       // queue it for the segmented build when its slots fit, else sort here
       if (threadIdx.x == 0) {
@@ -784,6 +865,7 @@ __global__ void k_huge_reset(HugeSort hs) {
   if (threadIdx.x == 0) {
     hs.ctl[0] = 0;
     hs.ctl[1] = 0;
+    hs.ctl[2] = 0;
   }
 }
 
@@ -853,6 +935,7 @@ __global__ void k_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tup
       actl->overflow = 0;
       actl->merge_overflow = 0;
       actl->filter_entries = 0;
+      actl->filter_bits = 0;
       actl->relocated = 0;
       actl->ndirty = 0;
       actl->nheavy = 0;
@@ -1812,7 +1895,7 @@ hipError_t launch_lbuild(const DevGraph &g, const uint8_t *lmap, const uint32_t 
                          const HugeSort &hs, hipStream_t s) {
   const uint32_t blocks = list ? 2048u : ((g.n + 3) / 4 < 8192 ? (g.n + 3) / 4 : 8192u);
   hipLaunchKernelGGL(k_huge_reset, dim3(1), dim3(64), 0, s, hs);
-  hipLaunchKernelGGL(k_lbuild, dim3(blocks), dim3(256), 0, s, g, lmap, list, count, hs.bucket_bytes);
+  hipLaunchKernelGGL(k_lbuild, dim3(blocks), dim3(256), 0, s, g, lmap, list, count, hs);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_lbuild_big, dim3(list ? 256u : 1024u), dim3(kLbuildWaves * 64), 0, s, g, lmap, list, count,

@@ -36,6 +36,7 @@ EXPORTS = (
     "dsm_match_batch", "dsm_process_batch", "dsm_get_tuples", "dsm_get_stats", "dsm_get_graph",
     "dsm_get_candidates", "dsm_get_query_time", "dsm_num_vertices", "dsm_num_queries", "dsm_file_read", "dsm_file_free",
     "dsm_file_graph", "dsm_file_query", "dsm_file_num_batches", "dsm_file_batch", "dsm_file_error",
+    "dsm_file_edge_labels",
     "dsm_server_create", "dsm_server_submit", "dsm_server_wait", "dsm_server_error", "dsm_server_destroy",
     "dsm_admit", "dsm_estimate_request_bytes", "dsm_autotune", "dsm_process_batches",
     "dsm_get_refined_candidates", "dsm_cluster_plan", "dsm_intersect_sorted", "dsm_intersect_sorted_noalloc",
@@ -94,7 +95,8 @@ class Stats(C.Structure):
                 ("dfs_entries", C.c_uint64), ("dfs_search_loads", C.c_uint64), ("dfs_launches", C.c_uint32),
                 ("dfs_merge_bytes", C.c_uint64), ("dfs_busy_ms", C.c_double), ("dfs_waves", C.c_uint64),
                 ("dfs_block_imbalance", C.c_double), ("dfs_head_idle", C.c_double), ("dfs_tail_idle", C.c_double),
-                ("ms_refine", C.c_float), ("filter_entries", C.c_uint64)]
+                ("ms_refine", C.c_float), ("filter_entries", C.c_uint64),
+                ("filter_bits", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -174,6 +176,8 @@ def lib():
     L.dsm_file_num_batches.argtypes = [vp]
     L.dsm_file_batch.restype = C.c_int
     L.dsm_file_batch.argtypes = [vp, C.c_uint32, C.POINTER(C.POINTER(Change)), C.POINTER(C.c_uint32)]
+    L.dsm_file_edge_labels.restype = C.c_int
+    L.dsm_file_edge_labels.argtypes = [vp, C.POINTER(C.POINTER(C.c_uint32))]
     L.dsm_file_error.restype = C.c_char_p
     L.dsm_file_error.argtypes = []
     L.dsm_server_create.restype = C.c_int
@@ -237,6 +241,17 @@ class DsmFile:
         eu = np.ctypeslib.as_array(C.cast(g.edge_src, C.POINTER(C.c_uint32)), (max(m, 1),))[:m].copy() if m else np.zeros(0, np.uint32)
         ev = np.ctypeslib.as_array(C.cast(g.edge_dst, C.POINTER(C.c_uint32)), (max(m, 1),))[:m].copy() if m else np.zeros(0, np.uint32)
         return n, lab, eu, ev
+
+    def edge_labels(self):
+        """The e lines' edge labels (0 where absent), or None when no line has one."""
+        p = C.POINTER(C.c_uint32)()
+        has = lib().dsm_file_edge_labels(self._h, C.byref(p))
+        if has < 0:
+            raise DsmError(has, "dsm_file_edge_labels")
+        if not has:
+            return None
+        m = self.graph()[2].size
+        return np.ctypeslib.as_array(p, (m,)).copy()
 
     @property
     def num_batches(self):

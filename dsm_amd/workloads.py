@@ -255,6 +255,26 @@ def small(seed=1):
     return _assemble("small", n, labels, eu, ev, keep, recs, 1, [q], seed + 40)
 
 
+def dense(seed=3):
+    """400 V / 8,000 uniform edges (mean degree 40), 3 uniform labels,
+    triangle + pendant; 120 inserts and 120 deletes interleaved, 8 records per
+    batch, so that most batches carry matches of both signs."""
+    L = _lib()
+    n, m = 400, 8000
+    eu = np.zeros(m, np.uint32)
+    ev = np.zeros(m, np.uint32)
+    assert L.gen_edges_uniform(n, m, seed, eu, ev) == m
+    labels = np.zeros(n, np.uint32)
+    L.gen_labels(n, 3, 0.0, seed + 10, labels)
+    keep, recs = _stream(m, 120, 120, 8, seed + 20, interleave=True)
+    off = np.zeros(n + 1, np.uint64)
+    nbr = np.zeros(2 * int(keep.sum()), np.uint32)
+    L.gen_csr(n, int(keep.sum()), eu[keep].copy(), ev[keep].copy(), off, nbr)
+    q = extract_query(n, off, nbr, labels, "tri_pendant", 4, seed + 30, min_label=0)
+    assert q is not None
+    return _assemble("dense", n, labels, eu, ev, keep, recs, 8, [q], seed + 40)
+
+
 _RMAT_CACHE = {}
 
 

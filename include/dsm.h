@@ -122,7 +122,9 @@ typedef struct {
   uint64_t tuples;                 /* tuples found (may exceed the capacity) */
   uint32_t touched_vertices;       /* vertices whose adjacency changed */
   uint32_t relabeled_vertices;     /* vertices whose label changed */
-  uint32_t dirty_groups;           /* 64-vertex groups re-filtered */
+  uint32_t dirty_groups;           /* vertices re-filtered (endpoints, relabelled vertices
+                                      and their neighbours); summed over the batches of
+                                      dsm_process_batches, like filter_entries */
   uint32_t relocated_vertices;     /* adjacency lists moved to a larger slot */
   float ms_apply, ms_filter, ms_seed, ms_dfs_pos, ms_dfs_neg; /* device time */
   uint64_t pool_entries_used, pool_entries_capacity;
@@ -147,6 +149,8 @@ typedef struct {
   float ms_refine;                 /* device time of the cluster-encoding refinement */
   uint64_t filter_entries;         /* list entries of the vertices whose candidacy was
                                       re-evaluated (the refilter's work measure) */
+  uint64_t filter_bits;            /* candidate-bitmap bits those vertices were given (one
+                                      per signature of their label) */
 } dsm_stats;
 
 /* Outcome of dsm_autotune: the launch parameters kept and the DFS time of the
@@ -362,7 +366,9 @@ uint64_t dsm_estimate_request_bytes(const dsm_engine *e, uint32_t num_changes);
 typedef struct dsm_file dsm_file;
 
 /* Parses a DSM text file (io.cu:122-252): `t n m`, n `v id label deg` lines in
- * id order, m `e a b` lines, then for data files `-` and batches `b bid k`
+ * id order, m `e a b` lines (each may carry an edge label as a third number,
+ * which the reference's format lacks; it is kept, see dsm_file_edge_labels,
+ * and the matching ignores it), then for data files `-` and batches `b bid k`
  * with k `a|d src dst label` lines, bid = 0,1,... in order. A v-line degree
  * that differs from the vertex's e-line count is DSM_EFORMAT (the reference
  * would build a corrupt CSR, io.cu:174). A data file that ends after its e
@@ -373,6 +379,10 @@ void dsm_file_free(dsm_file *f);
 /* Views into the parsed file; valid until dsm_file_free. */
 int dsm_file_graph(const dsm_file *f, dsm_graph_desc *graph);
 int dsm_file_query(const dsm_file *f, dsm_query_desc *query);
+/* Edge labels of the e lines, in file order (0 on lines without one): returns
+ * 1 if any e line carried one, else 0. No counterpart in the reference, whose
+ * reader has no edge labels; the engine's matching ignores them. */
+int dsm_file_edge_labels(const dsm_file *f, const uint32_t **labels);
 uint32_t dsm_file_num_batches(const dsm_file *f);
 int dsm_file_batch(const dsm_file *f, uint32_t b, const dsm_change **changes, uint32_t *num_changes);
 const char *dsm_file_error(void);

@@ -14,6 +14,12 @@
                                       sorted tuples of that workload from the
                                       CPU oracle, each record's count checked
                                       here against the brute-force enumerator
+  dense_expected.json                 the same for the 400-vertex, mean-degree
+                                      40 workload (8 interleaved records per
+                                      batch), with a digest of each batch's
+                                      tuples in place of the tuples; the graph
+                                      is regenerated from its seed by
+                                      dsm_amd.workloads.dense()
 
 Run from the repository root: python tests/golden/make_fixtures.py
 """
@@ -87,11 +93,55 @@ def small_fixture():
             "batches": batches, "final_graph": final}
 
 
+def tuple_digest(t):
+    """Order-independent digest of a set of tuples (rows): the sum over rows of
+    a 64-bit mix of the row, mod 2^61."""
+    t = np.asarray(t, np.uint64)
+    if t.size == 0:
+        return 0
+    t = t.reshape(len(t), -1)
+    h = np.zeros(len(t), np.uint64)
+    with np.errstate(over="ignore"):
+        for c in range(t.shape[1]):
+            h = (h ^ t[:, c]) * np.uint64(0x9E3779B97F4A7C15)
+            h ^= h >> np.uint64(29)
+    return int(sum(int(x) for x in h) % (1 << 61))
+
+
+def dense_fixture():
+    w = W.dense()
+    q = w.queries[0]
+    g = O.Graph(w.n, w.labels, w.eu, w.ev)
+    oq = O.Query(q.n, q.labels, q.eu, q.ev)
+    batches = []
+    for b in range(w.num_batches):
+        add, src, dst, lab = w.batch(b)
+        dels = add == 0
+        cand = g.candidates(oq)
+        neg_rec = g.count_records(oq, src[dels], dst[dels], cand)
+        for s_, d_, c in zip(src[dels], dst[dels], neg_rec):
+            assert c == g.brute_record(oq, s_, d_), "oracle != brute force (neg)"
+        neg_t = g.tuples_records(oq, src[dels], dst[dels], cand)
+        g.apply(add, src, dst, lab, relabel=True)
+        cand = g.candidates(oq)
+        pos_rec = g.count_records(oq, src[~dels], dst[~dels], cand)
+        for s_, d_, c in zip(src[~dels], dst[~dels], pos_rec):
+            assert c == g.brute_record(oq, s_, d_), "oracle != brute force (pos)"
+        pos_t = g.tuples_records(oq, src[~dels], dst[~dels], cand)
+        assert len(pos_t) == pos_rec.sum() and len(neg_t) == neg_rec.sum()
+        batches.append({"pos": int(pos_rec.sum()), "neg": int(neg_rec.sum()),
+                        "pos_digest": tuple_digest(pos_t), "neg_digest": tuple_digest(neg_t)})
+    return {"workload": "dense", "relabel": True, "query": {"labels": q.labels.tolist(), "edges": q.edges},
+            "batches": batches}
+
+
 def main():
     with open(os.path.join(HERE, "example_expected.json"), "w") as f:
         json.dump(EXAMPLE, f, indent=1)
     with open(os.path.join(HERE, "small_expected.json"), "w") as f:
         json.dump(small_fixture(), f)
+    with open(os.path.join(HERE, "dense_expected.json"), "w") as f:
+        json.dump(dense_fixture(), f)
     print("fixtures written")
 
 

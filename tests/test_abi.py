@@ -72,6 +72,47 @@ def test_static_format_is_a_graph_without_batches(golden_dir, tmp_path):
         assert f.num_batches == 0
 
 
+def test_edge_labels_are_an_optional_third_field(golden_dir, tmp_path):
+    """e lines may carry an edge label (the R-MAT workloads' 4 or 8 uniform
+    edge labels): on some lines or all, in data, static and query files, the
+    graph and the batches are those of the file without them, and the labels
+    read back in file order (0 where a line has none)."""
+    src = open(os.path.join(golden_dir, "small_g.in")).read().split("\n")
+    plain = E.DsmFile(os.path.join(golden_dir, "small_g.in"), False)
+    want = plain.graph()
+    assert plain.edge_labels() is None
+    rng = np.random.default_rng(7)
+    elines = [i for i, l in enumerate(src) if l.startswith("e ")]
+    for every in (1, 3):
+        lab = rng.integers(0, 8, len(elines))
+        out = list(src)
+        expect = []
+        for j, i in enumerate(elines):
+            if j % every == 0:
+                out[i] = "%s %d" % (src[i], lab[j])
+                expect.append(int(lab[j]))
+            else:
+                expect.append(0)
+        text = "\n".join(out)
+        for name, body, is_query in (("dyn.in", text, False), ("static.in", text[:text.index("-")], False),
+                                     ("query.in", text[:text.index("-")], True)):
+            p = tmp_path / name
+            p.write_text(body)
+            f = E.DsmFile(str(p), is_query)
+            g = f.graph()
+            assert g[0] == want[0] and all(x.tolist() == y.tolist() for x, y in zip(g[1:], want[1:]))
+            assert f.edge_labels().tolist() == expect
+            if name == "dyn.in":
+                assert f.num_batches == plain.num_batches
+                for b in range(f.num_batches):
+                    assert all(np.array_equal(x, y) for x, y in zip(f.batch(b), plain.batch(b)))
+    # an edge label cannot stand in for a missing endpoint
+    p = tmp_path / "short.in"
+    p.write_text("t 2 1\nv 0 0 1\nv 1 0 1\ne 0\n-\n")
+    with pytest.raises(E.DsmError):
+        E.DsmFile(str(p), False)
+
+
 BAD = {
     "no_header": "v 0 0 0\n",
     "degree_mismatch": "t 2 1\nv 0 0 2\nv 1 0 1\ne 0 1\n-\n",
@@ -141,3 +182,31 @@ def test_intersect_rejects_unsorted_input():
             E.intersect_sorted(a, b)
         assert ei.value.code == E.DSM_EINVAL
     assert E.intersect_sorted([], [1, 2]).size == 0
+
+
+STRUCTS = [("Change", "dsm_change"), ("GraphDesc", "dsm_graph_desc"), ("QueryDesc", "dsm_query_desc"),
+           ("Options", "dsm_options"), ("TuneReport", "dsm_tune_report"), ("Stats", "dsm_stats"),
+           ("ServerOptions", "dsm_server_options"), ("RequestInfo", "dsm_request_info")]
+
+
+def test_ctypes_structs_match_header_layout(tmp_path):
+    """Every ctypes mirror has the C struct's size and its fields' offsets, as
+    gcc lays out include/dsm.h."""
+    import ctypes
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dsm.h"', "int main(void) {"]
+    for py, c in STRUCTS:
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (py, c))
+        for f, _ in getattr(E, py)._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (py, f, c, f))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split("\n")
+    got = dict((" ".join(l.split()[:-1]), int(l.split()[-1])) for l in out if l)
+    for py, _ in STRUCTS:
+        cls = getattr(E, py)
+        assert got["%s size" % py] == ctypes.sizeof(cls), py
+        for f, _ in cls._fields_:
+            assert got["%s %s" % (py, f)] == getattr(cls, f).offset, (py, f)

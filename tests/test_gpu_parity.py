@@ -162,6 +162,31 @@ def test_small_workload_golden(golden_dir):
         eng.close()
 
 
+def test_dense_workload_golden(golden_dir):
+    """400 V, mean degree 40, triangle+pendant, 8 interleaved inserts and
+    deletes per batch (23 of 30 batches carry positive matches, 16 negative
+    ones): counts and a digest of each batch's tuples against the committed
+    fixture."""
+    import sys
+    sys.path.insert(0, golden_dir)
+    import make_fixtures as MF
+    say("case: dense workload golden fixture (30 batches of 8 records)")
+    exp = json.load(open(os.path.join(golden_dir, "dense_expected.json")))
+    w = W.dense()
+    q = w.queries[0]
+    assert q.labels.tolist() == exp["query"]["labels"]
+    eng = E.Engine((w.n, w.labels, w.eu, w.ev), [(q.n, q.labels, q.eu, q.ev)], tuple_capacity=1 << 16)
+    try:
+        for b, want in enumerate(exp["batches"]):
+            pos, neg = eng.process_batch(*w.batch(b))
+            assert (int(pos[0]), int(neg[0])) == (want["pos"], want["neg"]), "batch %d" % b
+            t = eng.tuples()
+            assert MF.tuple_digest(t[(0, 1)]) == want["pos_digest"], "batch %d pos tuples" % b
+            assert MF.tuple_digest(t[(0, 0)]) == want["neg_digest"], "batch %d neg tuples" % b
+    finally:
+        eng.close()
+
+
 # ------------------------------------------------------------ random streams
 
 @pytest.mark.parametrize("relabel", [True, False])
@@ -657,3 +682,75 @@ def test_insert_batch_larger_than_merge_scratch():
     dst = rng.integers(0, n, k).astype(np.uint32)
     big = (np.ones(k, np.uint8), src, dst, np.zeros(k, np.uint32))
     compare_stream(n, labels, eu, ev, queries, [big], relabel=False, check_graph=True)
+
+
+@pytest.mark.parametrize("overlap", [True, False])
+def test_hub_copy_current_after_relabel_to_used_label(overlap):
+    """Label-sorted copies and candidacy are kept only for vertices with a
+    label some query uses. Two hubs (degree 5,000, built by a block, and 300,
+    built by a wave) carry a label no query uses while their lists and their
+    neighbours' labels change; then inserts relabel them to used labels, and
+    their copies must be current from that batch on. Counts, graph and bitmaps
+    against the oracle."""
+    say("case: hubs with an unused label, relabelled to used labels (overlap %s)" % overlap)
+    rng = np.random.default_rng(5151)
+    n = 8000
+    labels = rng.integers(0, 3, n).astype(np.uint32)
+    labels[:2] = 3  # no query uses label 3
+    edges = set()
+    for hub, deg in ((0, 5000), (1, 300)):
+        for v in rng.choice(np.arange(2, n), deg, replace=False):
+            edges.add((hub, int(v)))
+    while len(edges) < 5300 + 24000:
+        a, b = map(int, rng.integers(2, n, 2))
+        if a != b:
+            edges.add((min(a, b), max(a, b)))
+    e = np.array(sorted(edges), np.uint32)
+    eu, ev = e[:, 0].copy(), e[:, 1].copy()
+    queries = [qtuple("triangle", [0, 1, 2]), qtuple("square", [0, 1, 0, 2]), qtuple("tri_pendant", [1, 0, 2, 2])]
+    present = set(edges)
+    nbr = {0: [v for (h, v) in edges if h == 0], 1: [v for (h, v) in edges if h == 1]}
+
+    def batch(records):
+        add, src, dst, lab = zip(*records)
+        return (np.array(add, np.uint8), np.array(src, np.uint32), np.array(dst, np.uint32),
+                np.array(lab, np.uint32))
+
+    def fresh(hub, k, lab):
+        out = []
+        while len(out) < k:
+            v = int(rng.integers(2, n))
+            if (hub, v) not in present and all(r[2] != v for r in out):
+                present.add((hub, v))
+                out.append((1, hub, v, lab))
+        return out
+
+    def neighbour_relabels(hub, k):
+        # inserts between the hub's neighbours and other vertices: the
+        # neighbours take used labels, so the hub's groups change
+        out, seen = [], set()
+        for v in rng.choice(nbr[hub], k, replace=False):
+            w = int(rng.integers(2, n))
+            key = (min(int(v), w), max(int(v), w))
+            if int(v) == w or key in present or key in seen:
+                continue
+            seen.add(key)
+            present.add(key)
+            out.append((1, int(v), w, int(rng.integers(0, 3))))
+        return out
+
+    batches = []
+    # 1-2: the hubs keep label 3 (their own inserts carry it) while their
+    # lists grow and their neighbours are relabelled
+    for _ in range(2):
+        recs = fresh(0, 40, 3) + fresh(1, 20, 3) + neighbour_relabels(0, 300) + neighbour_relabels(1, 60)
+        dels = [(0, h, v, 0) for h in (0, 1) for v in rng.choice(nbr[h], 10, replace=False)
+                if (h, int(v)) in present]
+        for _, h, v, _ in dels:
+            present.discard((h, int(v)))
+        batches.append(batch(recs + [(0, h, int(v), l) for _, h, v, l in dels]))
+    # 3: inserts relabel the hubs to used labels
+    batches.append(batch(fresh(0, 5, 1) + fresh(1, 5, 2) + neighbour_relabels(0, 100)))
+    # 4: more changes around the now-used hubs
+    batches.append(batch(fresh(0, 30, 1) + fresh(1, 30, 2) + neighbour_relabels(0, 200) + neighbour_relabels(1, 50)))
+    compare_stream(n, labels, eu, ev, queries, batches, relabel=True, check_graph=True, overlap_apply=overlap)

@@ -273,3 +273,36 @@ def test_multiset_over_records_counts_duplicates():
     q = O.Query(3, [0, 0, 0], [0, 1, 0], [1, 2, 2])  # triangle
     r = O.run_stream(g, [q], [([1, 1], [0, 0], [2, 2], [0, 0])], relabel=True)[0]
     assert r["pos"] == [2 * 6]  # 6 ordered embeddings of a triangle use edge (0,2)
+
+
+# ------------------------------------------------------ committed fixtures
+
+@pytest.mark.parametrize("name", ["small", "dense"])
+def test_oracle_reproduces_committed_fixtures(golden_dir, name):
+    """The oracle (each record's count checked against the brute force again
+    on the way) still writes the committed small and dense fixtures exactly."""
+    import sys
+    sys.path.insert(0, golden_dir)
+    import make_fixtures as MF
+    got = json.loads(json.dumps(getattr(MF, name + "_fixture")()))
+    want = json.load(open(os.path.join(golden_dir, name + "_expected.json")))
+    assert got == want
+
+
+def test_tuple_digest_is_order_free_and_sensitive():
+    """The digest of the dense fixture: row order does not matter; a changed
+    entry, a swapped pair inside a row or a dropped row changes it."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_fixtures as MF
+    t = np.array([[1, 2, 3, 4], [5, 6, 7, 8], [9, 10, 11, 12]], np.uint32)
+    d = MF.tuple_digest(t)
+    assert MF.tuple_digest(t[::-1]) == d
+    assert MF.tuple_digest(t[[0, 1]]) != d
+    u = t.copy()
+    u[1, 2] = 70
+    assert MF.tuple_digest(u) != d
+    u = t.copy()
+    u[0, [0, 1]] = u[0, [1, 0]]
+    assert MF.tuple_digest(u) != d
+    assert MF.tuple_digest(np.zeros((0, 4), np.uint32)) == 0
