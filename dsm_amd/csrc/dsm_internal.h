@@ -12,6 +12,9 @@ constexpr int kWave = 64;       // CDNA wavefront
 constexpr int kMaxRel = 32;     // labels that can appear in an NLF requirement
 constexpr int kMaxVertexChanges = 1024;  // per vertex per apply round (host splits beyond)
 constexpr uint8_t kNoLabel = 255;  // device label of vertices whose label no query uses
+#ifndef DSM_DONATE_BURST
+#define DSM_DONATE_BURST 4  // ranges a busy wave may hand out per check
+#endif
 #ifndef DSM_LABEL_SORTED
 #define DSM_LABEL_SORTED 64
 #endif
@@ -95,8 +98,14 @@ struct alignas(64) ApplyCtl {
   uint32_t merge_overflow;          // merges that found no room in the scratch (must stay 0)
   uint32_t filter_bits;             // candidate bits the re-filter writes (work measure)
   unsigned long long filter_entries;  // list entries of the vertices re-filtered (work measure)
+  // of the big lists above, those longer than kBlockList (split over the whole
+  // grid; queued from the back of the same arrays, the rest from the front)
+  uint32_t nrelabel_huge;
+  uint32_t nmerge_huge;
+  uint32_t ncommit_huge;
+  uint32_t pad[13];
 };
-static_assert(sizeof(ApplyCtl) == 64, "apply status layout");
+static_assert(sizeof(ApplyCtl) == 128, "apply status layout");
 
 struct DevGraph {
   uint32_t n;

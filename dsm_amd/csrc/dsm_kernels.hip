@@ -85,6 +85,30 @@ __device__ __forceinline__ void mark_dirty_wave(const DevGraph &g, const DirtyAr
 
 // ------------------------------------------------------------------ apply
 
+constexpr uint32_t kBlockList = 65536;
+
+// The lists of a *_big kernel: those of at most kBlockList entries sit at the
+// front of its array (nblock of them), a block each; the longer ones at the
+// back (nhuge, from index cap - 1 down), each split over every thread of the
+// grid. f(entry, first, stride) gets the thread's first index into the list
+// and the stride; both are multiples of 64 apart from the lane, so the lanes of
+// a wave step through the list together.
+template <typename F>
+__device__ __forceinline__ void for_big_lists(const uint32_t *arr, uint32_t cap, uint32_t nblock, uint32_t nhuge,
+                                              F f) {
+  for (uint32_t r = blockIdx.x; r < nblock; r += gridDim.x) f(arr[r], threadIdx.x, blockDim.x);
+  for (uint32_t r = 0; r < nhuge; r++)
+    f(arr[cap - 1 - r], blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// Queues entry v of a list of len entries on a *_big kernel's array: longer
+// than kBlockList at the back, else at the front.
+__device__ __forceinline__ void queue_big(uint32_t *arr, uint32_t cap, uint32_t *nblock, uint32_t *nhuge,
+                                          uint32_t len, uint32_t v) {
+  if (len > kBlockList) arr[cap - 1 - atomicAdd(nhuge, 1u)] = v;
+  else arr[atomicAdd(nblock, 1u)] = v;
+}
+
 // One wave per touched vertex: new list = (A \ D) u I written to scratch.
 // I and D are sorted and hold at most kMaxVertexChanges entries each; every
 // key has a single final operation, so I and D are disjoint.
@@ -150,7 +174,7 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
       if (lane == 0) {
         b.ndeg[t] = newdeg;
         b.soff[t] = so;
-        b.big_merge[atomicAdd(&actl->nmerge_big, 1u)] = t;
+        queue_big(b.big_merge, b.ntouched, &actl->nmerge_big, &actl->nmerge_huge, na, t);
       }
       wave_sync();
       continue;
@@ -177,35 +201,15 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
   }
 }
 
-// Work split for the lists of the *_big kernels: a list of at most
-// kBlockList entries is handled by one block (lists are dealt to blocks in
-// turn), a longer one by every thread of the grid. Returns the thread's first
-// index and stride in the list, or false when this block skips it.
-constexpr uint32_t kBlockList = 65536;
-__device__ __forceinline__ bool big_share(uint32_t r, uint32_t len, uint32_t &first, uint32_t &stride) {
-  if (len <= kBlockList) {
-    if (r % gridDim.x != blockIdx.x) return false;
-    first = threadIdx.x;
-    stride = blockDim.x;
-  } else {
-    first = blockIdx.x * blockDim.x + threadIdx.x;
-    stride = gridDim.x * blockDim.x;
-  }
-  return true;
-}
-
-// The merge copies of the lists longer than kApplyWave, each split evenly over
-// every thread of the grid, with the prefix counts k_apply_merge stored.
+// The merge copies of the lists longer than kApplyWave (for_big_lists splits
+// them), with the prefix counts k_apply_merge stored.
 __global__ __launch_bounds__(256) void k_apply_merge_big(DevGraph g, BatchDev b, uint32_t *scratch,
                                                          const ApplyCtl *actl) {
-  const uint32_t nbig = actl->nmerge_big;
   const uint32_t lane = lane_id();
-  for (uint32_t r = 0; r < nbig; r++) {
-    const uint32_t t = b.big_merge[r];
+  for_big_lists(b.big_merge, b.ntouched, actl->nmerge_big, actl->nmerge_huge, [&](uint32_t t, uint32_t tid,
+                                                                                   uint32_t nth) {
     const uint32_t x = b.tv[t];
     const uint32_t na = g.deg[x];
-    uint32_t tid, nth;
-    if (!big_share(r, na, tid, nth)) continue;
     // waves of the share: the thread's wave index and the number of waves
     const uint32_t gw = tid >> 6, nw = nth >> 6;
     const uint32_t *A = g.pool + g.start[x];
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(256) void k_apply_merge_big(DevGraph g, BatchDev b,
       const uint32_t p = lower_bound(D, nd, i);
       out[rr - dcum[p] + icum[j]] = i;
     }
-  }
+  });
 }
 
 // One wave per touched vertex: copy the merged list back into its slot, or
@@ -282,7 +286,7 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_commit(DevGraph g, B
     }
     if (nd > kApplyWave) {
       // k_apply_commit_big copies; the new degree is only read by later kernels
-      if (lane == 0) b.big_commit[atomicAdd(&actl->ncommit_big, 1u)] = t;
+      if (lane == 0) queue_big(b.big_commit, b.ntouched, &actl->ncommit_big, &actl->ncommit_huge, nd, t);
     } else {
       for (uint32_t j = lane; j < nd; j += 64) g.pool[dst + j] = scratch[so + j];
     }
@@ -294,25 +298,23 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_commit(DevGraph g, B
   }
 }
 
-// The commit copies of the lists longer than kApplyWave, split over the grid.
+// The commit copies of the lists longer than kApplyWave (for_big_lists splits
+// them).
 __global__ __launch_bounds__(256) void k_apply_commit_big(DevGraph g, BatchDev b, const uint32_t *scratch,
                                                           const ApplyCtl *actl) {
-  const uint32_t nbig = actl->ncommit_big;
-  for (uint32_t r = 0; r < nbig; r++) {
-    const uint32_t t = b.big_commit[r];
+  for_big_lists(b.big_commit, b.ntouched, actl->ncommit_big, actl->ncommit_huge, [&](uint32_t t, uint32_t first,
+                                                                                     uint32_t stride) {
     const uint32_t nd = b.ndeg[t];
-    uint32_t first, stride;
-    if (!big_share(r, nd, first, stride)) continue;
     const uint32_t dst = g.start[b.tv[t]];
     const unsigned long long so = b.soff[t];
     for (uint32_t j = first; j < nd; j += stride) g.pool[dst + j] = scratch[so + j];
-  }
+  });
 }
 
 __global__ void k_big_reset(ApplyCtl *actl, int merge, int commit) {
   if (threadIdx.x == 0) {
-    if (merge) actl->nmerge_big = 0;
-    if (commit) actl->ncommit_big = 0;
+    if (merge) actl->nmerge_big = actl->nmerge_huge = 0;
+    if (commit) actl->ncommit_big = actl->ncommit_huge = 0;
   }
 }
 
@@ -341,7 +343,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, BatchDev b, FilterT
     const uint32_t *A = g.pool + g.start[x];
     const uint32_t na = g.deg[x];
     if (na > kRelabelWave) {
-      if (lane == 0) b.rl_big[atomicAdd(&d.actl->nrelabel_big, 1u)] = x;
+      if (lane == 0) queue_big(b.rl_big, b.nrelabel, &d.actl->nrelabel_big, &d.actl->nrelabel_huge, na, x);
       continue;
     }
     for (uint32_t j0 = 0; j0 < na; j0 += 64) {
@@ -352,22 +354,18 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, BatchDev b, FilterT
 }
 
 // The neighbours of every relabelled vertex of degree > kRelabelWave, marked
-// dirty (big_share splits the lists).
+// dirty (for_big_lists splits the lists).
 __global__ __launch_bounds__(256) void k_relabel_big(DevGraph g, BatchDev b, DirtyArgs d) {
-  const uint32_t nbig = d.actl->nrelabel_big;
-  for (uint32_t r = 0; r < nbig; r++) {
-    const uint32_t x = b.rl_big[r];
+  for_big_lists(b.rl_big, b.nrelabel, d.actl->nrelabel_big, d.actl->nrelabel_huge, [&](uint32_t x, uint32_t first,
+                                                                                         uint32_t stride) {
     const uint32_t na = g.deg[x];
-    uint32_t first, stride;
-    if (!big_share(r, na, first, stride)) continue;
     const uint32_t *A = g.pool + g.start[x];
-    // stride is a multiple of 64 and first - lane_id() is the wave's start,
-    // so the lanes of a wave run the loop alike
+    // first - lane_id() is the wave's start, so its lanes run the loop alike
     for (uint32_t j0 = first - lane_id(); j0 < na; j0 += stride) {
       const uint32_t j = j0 + lane_id();
       mark_dirty_wave(g, d, j < na ? A[j] : 0u, j < na);
     }
-  }
+  });
 }
 
 // ----------------------------------------------------------------- filter
@@ -941,6 +939,7 @@ __global__ void k_ctl_reset(DfsCtl *ctl, ApplyCtl *actl, unsigned long long *tup
       actl->nheavy = 0;
       actl->relabeled = 0;
       actl->nrelabel_big = 0;
+      actl->nrelabel_huge = 0;
     }
   }
 }
@@ -1364,14 +1363,13 @@ __device__ void emit_tuples(const WaveHdr &S, const DfsArgs &a, uint32_t l, uint
 // more above the leaf; sixteen one level above it. The level being scanned
 // gives away work only when at least donate_min source entries remain. The
 // receiver re-fetches and re-filters the donated source range [mid, end).
-__device__ void try_donate(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l, uint32_t n) {
-  // waves waiting for work = tickets taken - slots reserved
-  uint32_t waiting = 0;
-  if (lane_id() == 0) {
-    const uint32_t h = ld_relaxed(&a.ctl->q_head.v), t = ld_relaxed(&a.ctl->q_tail.v);
-    waiting = (h > t && t < a.qcap) ? h - t : 0u;
-  }
-  if (bcast(waiting) == 0) return;
+// Hands ranges of the wave's unscanned work to waiting waves: up to
+// min(waiting, kDonateBurst) of them per call, each half of what is left at
+// the shallowest level that still holds enough, so that when most of the grid
+// runs dry at once the work spreads in fewer check periods.
+constexpr uint32_t kDonateBurst = DSM_DONATE_BURST;
+
+__device__ __forceinline__ bool donate_one(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l, uint32_t n) {
   uint32_t lv = 0xFFFFFFFFu, mid = 0, keep_cn = 0;
   for (uint32_t k = L0; k <= l; k++) {
     const uint32_t p = S.pos[k], e = S.end[k];
@@ -1411,7 +1409,7 @@ __device__ void try_donate(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l
       break;
     }
   }
-  if (lv == 0xFFFFFFFFu) return;
+  if (lv == 0xFFFFFFFFu) return false;
   const uint32_t e = S.end[lv];
   uint32_t slot = 0, ok = 0;
   if (lane_id() == 0) {
@@ -1423,7 +1421,7 @@ __device__ void try_donate(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l
     }
   }
   ok = bcast(ok);
-  if (!ok) return;
+  if (!ok) return false;
   slot = bcast(slot);
   const uint32_t lane = lane_id();
   uint32_t word = 0;
@@ -1444,6 +1442,19 @@ __device__ void try_donate(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l
     S.end[lv] = mid;
   }
   wave_sync();
+  return true;
+}
+
+__device__ void try_donate(WaveHdr &S, const DfsArgs &a, uint32_t L0, uint32_t l, uint32_t n) {
+  // waves waiting for work = tickets taken - slots reserved
+  uint32_t waiting = 0;
+  if (lane_id() == 0) {
+    const uint32_t h = ld_relaxed(&a.ctl->q_head.v), t = ld_relaxed(&a.ctl->q_tail.v);
+    waiting = (h > t && t < a.qcap) ? h - t : 0u;
+  }
+  waiting = bcast(waiting);
+  for (uint32_t k = 0; k < waiting && k < kDonateBurst; k++)
+    if (!donate_one(S, a, L0, l, n)) break;
 }
 
 template <int WPB, bool TUP>
