@@ -47,7 +47,47 @@ struct QueryHost {
   uint32_t n = 0;
   uint32_t lab[kMaxQ] = {};
   uint32_t adj[kMaxQ] = {};  // bitmask
+  uint32_t order_variant = 0;  // rule of its matching orders (next_in_order)
 };
+
+constexpr uint32_t kOrderVariants = 4;
+
+// The next query vertex of a matching order, among the unplaced vertices with
+// an earlier neighbour, by rule `variant` (ties: lowest id). An order never
+// changes a count, only the work: the tuner picks the rule per query.
+//   0: most earlier neighbours, then the rarest label in the data graph, then
+//      the highest query degree;
+//   1: most earlier neighbours, then the highest query degree, then the
+//      rarest label;
+//   2: the rarest label, then most earlier neighbours, then the highest degree;
+//   3: most earlier neighbours, then the most frequent label, then the highest
+//      query degree.
+uint32_t next_in_order(const QueryHost &Q, uint32_t placed, const std::vector<uint64_t> &label_freq,
+                       uint32_t variant) {
+  int best = -1;
+  int64_t bk[3] = {0, 0, 0};
+  for (uint32_t w = 0; w < Q.n; w++) {
+    if (placed >> w & 1) continue;
+    const int64_t b = __builtin_popcount(Q.adj[w] & placed);
+    if (b == 0) continue;
+    const int64_t dg = __builtin_popcount(Q.adj[w]);
+    const int64_t f = Q.lab[w] < label_freq.size() ? (int64_t)label_freq[Q.lab[w]] : 0;
+    int64_t k[3];
+    switch (variant) {
+      case 1: k[0] = b; k[1] = dg; k[2] = -f; break;
+      case 2: k[0] = -f; k[1] = b; k[2] = dg; break;
+      case 3: k[0] = b; k[1] = f; k[2] = dg; break;
+      default: k[0] = b; k[1] = -f; k[2] = dg; break;
+    }
+    if (best < 0 || k[0] > bk[0] || (k[0] == bk[0] && (k[1] > bk[1] || (k[1] == bk[1] && k[2] > bk[2])))) {
+      best = (int)w;
+      bk[0] = k[0];
+      bk[1] = k[1];
+      bk[2] = k[2];
+    }
+  }
+  return (uint32_t)best;
+}
 
 template <typename T>
 struct DevBuf {
@@ -110,6 +150,7 @@ struct dsm_engine {
   std::vector<Plan> plans;
   std::vector<uint32_t> plan_key;
   uint32_t max_plans_per_key = 0;
+  std::vector<uint64_t> label_freq;  // data vertices per label (orders the matching only)
 
   // device graph
   DevBuf<uint32_t> pool, start, deg, cap;
@@ -418,6 +459,62 @@ int build_cluster_ops(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq) {
   return DSM_OK;
 }
 
+// The plans: every query edge in both orientations, each with its matching
+// order (the query's order rule), grouped by the seed edge's label pair.
+void build_plans(dsm_engine *e) {
+  // plans: every query edge in both orientations
+  struct PK {
+    uint32_t key;
+    Plan p;
+  };
+  std::vector<PK> pk;
+  const uint32_t nq = (uint32_t)e->queries.size();
+  for (uint32_t q = 0; q < nq; q++) {
+    const QueryHost &Q = e->queries[q];
+    for (uint32_t u = 0; u < Q.n; u++)
+      for (uint32_t v = 0; v < Q.n; v++) {
+        if (!(Q.adj[u] >> v & 1)) continue;
+        Plan P{};
+        P.n = (uint8_t)Q.n;
+        P.query = (uint16_t)q;
+        uint32_t order[kMaxQ];
+        order[0] = u;
+        order[1] = v;
+        uint32_t placed = (1u << u) | (1u << v);
+        for (uint32_t k = 2; k < Q.n; k++) {
+          order[k] = next_in_order(Q, placed, e->label_freq, Q.order_variant);
+          placed |= 1u << order[k];
+        }
+        for (uint32_t k = 0; k < Q.n; k++) {
+          P.qv[k] = (uint8_t)order[k];
+          P.sig[k] = (uint16_t)e->qref[q][order[k]];
+          uint16_t back = 0, inj = 0;
+          for (uint32_t j = 0; j < k; j++) {
+            if (Q.adj[order[k]] >> order[j] & 1) back |= (uint16_t)(1u << j);
+            if (Q.lab[order[j]] == Q.lab[order[k]]) inj |= (uint16_t)(1u << j);
+          }
+          P.back[k] = back;
+          P.inj[k] = inj;
+          P.lix[k] = e->lmap[Q.lab[order[k]]];
+        }
+        pk.push_back({(Q.lab[u] << 8) | Q.lab[v], P});
+      }
+  }
+  std::stable_sort(pk.begin(), pk.end(), [](const PK &a, const PK &b) { return a.key < b.key; });
+  e->plans.clear();
+  e->plan_key.clear();
+  e->max_plans_per_key = 0;
+  for (size_t i = 0; i < pk.size();) {
+    size_t j = i;
+    while (j < pk.size() && pk[j].key == pk[i].key) j++;
+    e->max_plans_per_key = std::max<uint32_t>(e->max_plans_per_key, (uint32_t)(j - i));
+    i = j;
+  }
+  for (auto &x : pk) {
+    e->plans.push_back(x.p);
+    e->plan_key.push_back(x.key);
+  }}
+
 // label_freq[l]: data vertices of label l (l < 255), used only to order the
 // matching: it never changes a count.
 int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq, const std::vector<uint64_t> &label_freq) {
@@ -513,72 +610,9 @@ int build_queries(dsm_engine *e, const dsm_query_desc *qs, uint32_t nq, const st
   if (e->cluster) {
     if (int rc = build_cluster_ops(e, qs, nq)) return rc;
   }
-  // plans: every query edge in both orientations
-  struct PK {
-    uint32_t key;
-    Plan p;
-  };
-  std::vector<PK> pk;
-  for (uint32_t q = 0; q < nq; q++) {
-    const QueryHost &Q = e->queries[q];
-    for (uint32_t u = 0; u < Q.n; u++)
-      for (uint32_t v = 0; v < Q.n; v++) {
-        if (!(Q.adj[u] >> v & 1)) continue;
-        Plan P{};
-        P.n = (uint8_t)Q.n;
-        P.query = (uint16_t)q;
-        uint32_t order[kMaxQ];
-        order[0] = u;
-        order[1] = v;
-        uint32_t placed = (1u << u) | (1u << v);
-        for (uint32_t k = 2; k < Q.n; k++) {
-          // most earlier neighbours, then the rarest label in the data graph,
-          // then highest query degree, then lowest id
-          int best = -1, bb = -1, bd = -1;
-          uint64_t bf = ~0ull;
-          for (uint32_t w = 0; w < Q.n; w++) {
-            if (placed >> w & 1) continue;
-            const int b = __builtin_popcount(Q.adj[w] & placed), dg = __builtin_popcount(Q.adj[w]);
-            const uint64_t f = Q.lab[w] < label_freq.size() ? label_freq[Q.lab[w]] : 0;
-            if (b > bb || (b == bb && (f < bf || (f == bf && dg > bd)))) {
-              best = (int)w;
-              bb = b;
-              bf = f;
-              bd = dg;
-            }
-          }
-          order[k] = (uint32_t)best;
-          placed |= 1u << best;
-        }
-        for (uint32_t k = 0; k < Q.n; k++) {
-          P.qv[k] = (uint8_t)order[k];
-          P.sig[k] = (uint16_t)e->qref[q][order[k]];
-          uint16_t back = 0, inj = 0;
-          for (uint32_t j = 0; j < k; j++) {
-            if (Q.adj[order[k]] >> order[j] & 1) back |= (uint16_t)(1u << j);
-            if (Q.lab[order[j]] == Q.lab[order[k]]) inj |= (uint16_t)(1u << j);
-          }
-          P.back[k] = back;
-          P.inj[k] = inj;
-          P.lix[k] = e->lmap[Q.lab[order[k]]];
-        }
-        pk.push_back({(Q.lab[u] << 8) | Q.lab[v], P});
-      }
-  }
-  std::stable_sort(pk.begin(), pk.end(), [](const PK &a, const PK &b) { return a.key < b.key; });
-  e->plans.clear();
-  e->plan_key.clear();
-  e->max_plans_per_key = 0;
-  for (size_t i = 0; i < pk.size();) {
-    size_t j = i;
-    while (j < pk.size() && pk[j].key == pk[i].key) j++;
-    e->max_plans_per_key = std::max<uint32_t>(e->max_plans_per_key, (uint32_t)(j - i));
-    i = j;
-  }
-  for (auto &x : pk) {
-    e->plans.push_back(x.p);
-    e->plan_key.push_back(x.key);
-  }
+  e->label_freq = label_freq;
+  build_plans(e);
+
   return DSM_OK;
 }
 
@@ -1914,6 +1948,67 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     c.pl = !best.pl;
     if ((rc = consider(c))) return rc;
   }
+  // 8. matching order per query. The queries' plans are rebuilt with each
+  // order rule in turn (all queries at once: each query's task time is
+  // measured on its own), in the order 0..3 then 3..0 so that a drift of the
+  // clocks hits every rule alike; a query keeps the rule with the least task
+  // time if it beats its current rule by 5%. The new orders are kept only if
+  // the sample round is then faster in three interleaved pairs.
+  uint32_t order_changed = 0;
+  {
+    auto use_rules = [&](const std::vector<uint32_t> &rule) -> int {
+      for (uint32_t q = 0; q < nq; q++) e->queries[q].order_variant = rule[q];
+      build_plans(e);
+      HIPCHK(hipMemcpy(e->d_plans.p, e->plans.data(), e->plans.size() * sizeof(Plan), hipMemcpyHostToDevice));
+      return DSM_OK;
+    };
+    auto take_query_ms = [&](std::vector<double> *acc) -> int {
+      std::vector<double> ms(nq);
+      if (int r2 = dsm_get_query_time(e, ms.data(), nq)) return r2;
+      if (acc)
+        for (uint32_t q = 0; q < nq; q++) (*acc)[q] += ms[q];
+      return DSM_OK;
+    };
+    std::vector<uint32_t> cur(nq);
+    for (uint32_t q = 0; q < nq; q++) cur[q] = e->queries[q].order_variant;
+    std::vector<std::vector<double>> qms(kOrderVariants, std::vector<double>(nq, 0.0));
+    if ((rc = take_query_ms(nullptr))) return rc;
+    for (uint32_t pass = 0; pass < 2; pass++)
+      for (uint32_t i = 0; i < kOrderVariants; i++) {
+        const uint32_t v = pass ? kOrderVariants - 1 - i : i;
+        if ((rc = use_rules(std::vector<uint32_t>(nq, v)))) return rc;
+        float t = 0;
+        if ((rc = timed(best, t))) return rc;
+        if ((rc = take_query_ms(&qms[v]))) return rc;
+      }
+    std::vector<uint32_t> pick = cur;
+    for (uint32_t q = 0; q < nq; q++) {
+      double bt = qms[cur[q]][q];
+      for (uint32_t v = 0; v < kOrderVariants; v++)
+        if (qms[v][q] < 0.95 * qms[cur[q]][q] && qms[v][q] < bt) {
+          bt = qms[v][q];
+          pick[q] = v;
+        }
+    }
+    if (pick != cur) {
+      float t_cur = 0, t_new = 0;
+      for (int r = 0; r < 3; r++) {
+        float t = 0;
+        if ((rc = use_rules(cur)) || (rc = timed(best, t))) return rc;
+        t_cur += t;
+        if ((rc = use_rules(pick)) || (rc = timed(best, t))) return rc;
+        t_new += t;
+      }
+      if (t_new < t_cur) {
+        for (uint32_t q = 0; q < nq; q++) order_changed += pick[q] != cur[q];
+      } else if ((rc = use_rules(cur))) {
+        return rc;
+      }
+    } else if ((rc = use_rules(cur))) {
+      return rc;
+    }
+    if ((rc = take_query_ms(nullptr))) return rc;
+  }
   // the kept configuration, timed once more
   best_ms = 1e30f;
   for (int r = 0; r < 2; r++) {
@@ -1931,6 +2026,7 @@ int dsm_autotune(dsm_engine *e, const dsm_change *sample, uint32_t k, dsm_tune_r
     rep->donate_every = best.dev;
     rep->leaf_budget = best.lb;
     rep->bit_probe_last = best.pl ? 1 : 0;
+    rep->order_changed = order_changed;
     rep->trials = trials;
     rep->ms_initial = ms0;
     rep->ms_best = best_ms;

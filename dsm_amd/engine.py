@@ -79,7 +79,7 @@ class TuneReport(C.Structure):
                 ("lane_work", C.c_uint32), ("leaf_sets", C.c_int), ("leaf_segments", C.c_int),
                 ("donate_every", C.c_uint32), ("trials", C.c_uint32),
                 ("ms_initial", C.c_float), ("ms_best", C.c_float), ("leaf_budget", C.c_uint32),
-                ("bit_probe_last", C.c_int)]
+                ("bit_probe_last", C.c_int), ("order_changed", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

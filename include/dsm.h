@@ -163,6 +163,7 @@ typedef struct {
   float ms_initial, ms_best;
   uint32_t leaf_budget;
   int bit_probe_last;
+  uint32_t order_changed; /* queries whose matching-order rule the tuner changed */
 } dsm_tune_report;
 
 /* ------------------------------------------------------------ engine life */
@@ -305,7 +306,9 @@ uint32_t dsm_num_queries(const dsm_engine *e);
  * waves per block, blocks per CU, the donation threshold, the lane work limit
  * and leaf sets on or off, by coordinate descent over timed DFS rounds that
  * match the sample's records in the current graph (read-only, as negatives
- * are). Every configuration must give the same counts (else DSM_EDEVICE).
+ * are), and then each query's matching-order rule from the queries' own task
+ * times under every rule. Every configuration must give the same counts (else
+ * DSM_EDEVICE).
  * Keeps the fastest; changes no graph state but overwrites the counts and
  * statistics of the last batch. Call between batches. No counterpart in the
  * reference, whose launch shapes are constants (gpu_match.cu:1121-1431). */
