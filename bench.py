@@ -84,12 +84,23 @@ def qtuples(w):
     return [(q.n, q.labels, q.eu, q.ev) for q in w.queries]
 
 
-def tune_sample(w, nb=4):
-    """Records of the first nb batches: matched read-only by the tuner, so the
-    deletes (edges of the current graph) seed DFS roots and the inserts, absent
-    until applied, seed none."""
-    parts = [w.batch(b) for b in range(min(nb, w.num_batches))]
-    return tuple(np.concatenate([p[i] for p in parts]) for i in range(4))
+TUNE_SAMPLE = {"kind": "edges", "records": 16384}
+
+
+def tune_sample(w):
+    """The tuner's sample, matched read-only (every record that is an edge of
+    the current graph seeds DFS roots, as a delete would). "edges": random
+    edges of the initial graph, drawn with a fixed seed and independent of the
+    stream's batches (the stream's inserts and deletes are random edges of the
+    same graph); "batches": the records of the first 4 batches."""
+    if TUNE_SAMPLE["kind"] == "batches":
+        parts = [w.batch(b) for b in range(min(4, w.num_batches))]
+        return tuple(np.concatenate([p[i] for p in parts]) for i in range(4))
+    rng = np.random.default_rng(777)
+    k = min(TUNE_SAMPLE["records"], len(w.eu))
+    idx = rng.choice(len(w.eu), k, replace=False)
+    z = np.zeros(k, np.uint32)
+    return np.zeros(k, np.uint8), w.eu[idx].astype(np.uint32), w.ev[idx].astype(np.uint32), z
 
 
 def _agg_add(agg, st):
@@ -438,8 +449,12 @@ def main():
                     help="skip the leg that times the same stream with the cluster-encoding refinement")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-query plain leg (rmat1m: one 6-vertex query, one batch per call)")
+    ap.add_argument("--tune-sample", default="edges", choices=["edges", "batches"],
+                    help="the autotuner's sample: 16,384 random edges of the initial graph, or the records of "
+                         "the first 4 batches")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run one D2D copy of this many bytes")
     args = ap.parse_args()
+    TUNE_SAMPLE["kind"] = args.tune_sample
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
@@ -540,7 +555,9 @@ def main():
                       "query_selection": QUERY_RULE.get(w.name, ""),
                       "cluster_filter": bool(args.cluster_filter),
                       "mode": "batched, pipelined (a batch's positives share a launch with the next batch's negatives; the next batch is applied to a second copy of the graph meanwhile)",
-                      "parallelism": "single GPU", "tuned": bat["tuned"]},
+                      "parallelism": "single GPU", "tuned": bat["tuned"],
+                      "tune_sample": ("%d random edges of the initial graph (seed 777)" % TUNE_SAMPLE["records"]
+                                      if TUNE_SAMPLE["kind"] == "edges" else "records of the first 4 batches")},
            # pipelined batches complete one after another: the gap between two
            # batches' counts, not a batch's latency (a batch's positives finish
            # in the next batch's launch); per-batch latency is the one batch per
