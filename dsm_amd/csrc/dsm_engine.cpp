@@ -158,6 +158,7 @@ struct dsm_engine {
   DevBuf<uint32_t> lpool, loff;  // label-sorted copies of long lists
   DevBuf<uint32_t> huge_list, huge_ctl, huge_hist;  // segmented builds of the longest lists
   DevBuf<uint32_t> big_list;                         // vertices whose copy a whole block builds
+  DevBuf<uint32_t> mid_list;                         // vertices whose copy one wave builds
   uint32_t huge_segs = 0;
   uint32_t nbuckets = 1;
   unsigned long long pool_cap = 0;
@@ -306,6 +307,8 @@ struct dsm_engine {
     h.ctl = huge_ctl.p;
     h.big = big_list.p;
     h.big_cap = (uint32_t)big_list.n;
+    h.mid = mid_list.p;
+    h.mid_cap = (uint32_t)mid_list.n;
     h.hist = huge_hist.p;
     h.segs = huge_segs;
     // the merge scratch (at least one entry per pool entry) is idle while the
@@ -628,7 +631,9 @@ int size_huge_sort(dsm_engine *e) {
   }
   if (!e->huge_list.p) {
     HIPCHK(e->huge_list.alloc(2 * 4096));
-    HIPCHK(e->huge_ctl.alloc(3));
+    HIPCHK(e->huge_ctl.alloc(4));
+    // every vertex at most once (a dirty list holds each vertex once)
+    HIPCHK(e->mid_list.alloc(e->n));
     // at most 2 m / 4,097 vertices have a degree above 4,096; a longer queue
     // makes k_lbuild_big look at every listed vertex instead
     HIPCHK(e->big_list.alloc((size_t)e->n / 64 + 4096));

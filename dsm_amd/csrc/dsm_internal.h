@@ -335,9 +335,11 @@ constexpr uint32_t kHugeSeg = 65536;
 struct HugeSort {
   uint32_t *list;      // [cap][2]: vertex, first segment slot
   uint32_t cap;
-  uint32_t *ctl;       // [3]: queued vertices, segment slots taken, vertices on big
+  uint32_t *ctl;       // [4]: queued vertices, segment slots taken, vertices on big, on mid
   uint32_t *big;       // [big_cap]: vertices of degree > 4,096 for the block-per-list build
   uint32_t big_cap;
+  uint32_t *mid;       // [mid_cap]: vertices of degree 65..4,096 for the wave-per-list build
+  uint32_t mid_cap;
   uint32_t *hist;      // [segs][kMaxRel + 2]
   uint32_t segs;
   // one byte per pool entry (indexed like the pool): the bucket of each entry,

@@ -609,47 +609,57 @@ __device__ __forceinline__ void build_one(const DevGraph &g, const uint8_t *s_lm
   bucket_scatter(g, s_lmap, A, 0, d, bits, cnt, g.lpool + g.start[v], bk);
 }
 
-// The listed vertices (or all), a lane each: those of degree in
-// (kLabelSorted, kLbuildBlock] are built here, a wave per vertex in turn; the
-// longer ones are queued on hs.big for k_lbuild_big. Vertices whose label no
-// query uses are skipped (never read, see mark_dirty_used); a full build
-// leaves their groups empty.
-__global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap, const uint32_t *list,
-                                                const uint32_t *count, HugeSort hs) {
+// Appends the lanes' vertices with `take` to q (count at *cnt, room for cap),
+// one counter add per wave.
+__device__ __forceinline__ void queue_wave(uint32_t *q, uint32_t cap, uint32_t *cnt, uint32_t v, bool take) {
+  const uint64_t m = ballot(take);
+  if (m == 0) return;
+  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+  uint32_t at = 0;
+  if (lane_id() == leader) at = atomicAdd(cnt, (uint32_t)__popcll(m));
+  at = __shfl(at, (int)leader) + mbcnt(m);
+  if (take && at < cap) q[at] = v;
+}
+
+// The listed vertices (or all), a lane each, sorted into the two build
+// kernels' queues: degree in (kLabelSorted, kLbuildBlock] on hs.mid for
+// k_lbuild (a wave each), longer on hs.big for k_lbuild_big (a block each).
+// Vertices whose label no query uses are skipped (never read, see
+// mark_dirty_used); a full build leaves their groups empty.
+__global__ __launch_bounds__(256) void k_lbuild_pick(DevGraph g, const uint32_t *list, const uint32_t *count,
+                                                     HugeSort hs) {
+  const uint32_t nb = g.nbuckets;
+  const uint32_t total = list ? (*count < g.n ? *count : g.n) : g.n;
+  for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < total;
+       base += gridDim.x * blockDim.x) {
+    const uint32_t t = base + lane_id();
+    uint32_t v = 0, d = 0;
+    bool used = false;
+    if (t < total) {
+      v = list ? list[t] : t;
+      d = g.deg[v];
+      used = g.lab[v] != kNoLabel;
+      if (!used && !list && d > kLabelSorted)
+        for (uint32_t k = 0; k <= nb; k++) g.loff[(size_t)v * (nb + 1) + k] = 0;
+    }
+    queue_wave(hs.mid, hs.mid_cap, &hs.ctl[3], v, used && d > kLabelSorted && d <= kLbuildBlock);
+    queue_wave(hs.big, hs.big_cap, &hs.ctl[2], v, used && d > kLbuildBlock);
+  }
+}
+
+// A wave per vertex queued on hs.mid: its label-sorted copy (build_one).
+__global__ __launch_bounds__(256) void k_lbuild(DevGraph g, const uint8_t *lmap, HugeSort hs) {
   __shared__ uint8_t s_lmap[256];
   __shared__ uint32_t s_cnt[4][kMaxRel + 2];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = lmap[i];
   __syncthreads();
-  const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t wv = threadIdx.x >> 6;
   const uint32_t nb = g.nbuckets;
   const uint32_t bits = 32u - (uint32_t)__builtin_clz(nb > 1 ? nb - 1 : 1u);
-  const uint32_t total = list ? (*count < g.n ? *count : g.n) : g.n;
-  uint32_t *cnt = s_cnt[wv];
-  for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base < total; base += gridDim.x * 256) {
-    const uint32_t t = base + lane;
-    uint32_t vl = 0, dl = 0;
-    bool used = false;
-    if (t < total) {
-      vl = list ? list[t] : t;
-      dl = g.deg[vl];
-      used = g.lab[vl] != kNoLabel;
-      if (!used && !list && dl > kLabelSorted)
-        for (uint32_t k = 0; k <= nb; k++) g.loff[(size_t)vl * (nb + 1) + k] = 0;
-    }
-    const uint64_t big = ballot(used && dl > kLbuildBlock);
-    if (big) {
-      const uint32_t leader = (uint32_t)__builtin_ctzll(big);
-      uint32_t at = 0;
-      if (lane == leader) at = atomicAdd(&hs.ctl[2], (uint32_t)__popcll(big));
-      at = __shfl(at, (int)leader) + mbcnt(big);
-      if (((big >> lane) & 1) && at < hs.big_cap) hs.big[at] = vl;
-    }
-    uint64_t mid = ballot(used && dl > kLabelSorted && dl <= kLbuildBlock);
-    while (mid) {
-      const uint32_t k = (uint32_t)__builtin_ctzll(mid);
-      mid &= mid - 1;
-      build_one(g, s_lmap, __shfl(vl, (int)k), __shfl(dl, (int)k), nb, bits, cnt, hs.bucket_bytes);
-    }
+  const uint32_t total = hs.ctl[3] < hs.mid_cap ? hs.ctl[3] : hs.mid_cap;
+  for (uint32_t t = blockIdx.x * 4 + wv; t < total; t += gridDim.x * 4) {
+    const uint32_t v = hs.mid[t];
+    build_one(g, s_lmap, v, g.deg[v], nb, bits, s_cnt[wv], hs.bucket_bytes);
   }
 }
 
@@ -864,6 +874,7 @@ __global__ void k_huge_reset(HugeSort hs) {
     hs.ctl[0] = 0;
     hs.ctl[1] = 0;
     hs.ctl[2] = 0;
+    hs.ctl[3] = 0;
   }
 }
 
@@ -1906,9 +1917,11 @@ hipError_t launch_lbuild(const DevGraph &g, const uint8_t *lmap, const uint32_t 
                          const HugeSort &hs, hipStream_t s) {
   const uint32_t blocks = list ? 2048u : ((g.n + 3) / 4 < 8192 ? (g.n + 3) / 4 : 8192u);
   hipLaunchKernelGGL(k_huge_reset, dim3(1), dim3(64), 0, s, hs);
-  hipLaunchKernelGGL(k_lbuild, dim3(blocks), dim3(256), 0, s, g, lmap, list, count, hs);
+  hipLaunchKernelGGL(k_lbuild_pick, dim3(blocks), dim3(256), 0, s, g, list, count, hs);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lbuild, dim3(2048), dim3(256), 0, s, g, lmap, hs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_lbuild_big, dim3(list ? 256u : 1024u), dim3(kLbuildWaves * 64), 0, s, g, lmap, list, count,
                      hs);
   if ((e = hipGetLastError()) != hipSuccess) return e;
