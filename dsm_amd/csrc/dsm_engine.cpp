@@ -52,6 +52,13 @@ struct QueryHost {
 
 constexpr uint32_t kOrderVariants = 4;
 
+#ifndef DSM_DELTA_SYNC
+#define DSM_DELTA_SYNC 1
+#endif
+// 1: an overlapped update brings the second copy of the graph up to date by
+// copying only what the previous update changed (catch_up)
+constexpr bool kDeltaSync = DSM_DELTA_SYNC;
+
 // The next query vertex of a matching order, among the unplaced vertices with
 // an earlier neighbour, by rule `variant` (ties: lowest id). An order never
 // changes a count, only the work: the tuner picks the rule per query.
@@ -215,6 +222,14 @@ struct dsm_engine {
   hipEvent_t ev_pre = nullptr;  // all work on stream before a pipelined round
   bool overlap_apply = true;
   bool lpool_stale = false;  // the pool grew during the current update
+  // The second copy of the graph differs from the current one only by the
+  // last update, whose touched and relabelled vertices are kept here (its
+  // dirty vertices stay on dirty_list): the next overlapped update copies just
+  // those instead of the whole graph. Any other change clears it.
+  bool shadow_delta_ok = false;
+  bool grew = false;  // the pool grew during the update being applied
+  std::vector<uint32_t> last_touched, last_relabel;
+  DevBuf<uint32_t> d_last_touched, d_last_relabel, sync_q, sync_ctl;
   int cu_lds = 160 * 1024;   // LDS bytes per CU
   // per-wave task time of the last DFS launch (balance statistics)
   DevBuf<unsigned long long> wave_busy;
@@ -1142,6 +1157,7 @@ int grow_pool(dsm_engine *e, unsigned long long extra) {
   // scratch, which still holds the lists of the commits that failed)
   HIPCHK(e->lpool.alloc(ncap));
   e->lpool_stale = true;
+  e->grew = true;
   // scratch must hold one copy of every list; the failed commits' lists move
   // with it
   if (e->scratch_cap < ncap + (1ull << 20)) {
@@ -1275,6 +1291,8 @@ void take_refine_time(dsm_engine *e) {
 // end.
 int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d) {
   int rc;
+  e->shadow_delta_ok = false;
+  e->grew = false;
   HIPCHK(launch_ctl_reset(nullptr, e->actl.p, nullptr, nullptr, 0, e->stream));
   if ((rc = do_apply_delta(e, d))) return rc;
   // the dirty list (touched vertices, relabelled ones and their neighbours) is
@@ -1452,6 +1470,46 @@ int copy_graph(dsm_engine *e, const dsm_engine::GraphCopy &c) {
 // enqueued before the round) and takes the batch, all on stream2. Returns
 // once the update is done; the copies stay swapped, or with keep = false
 // swap back (the update is dropped: the tuner's load).
+DevGraph graph_of(const dsm_engine *e, const dsm_engine::GraphCopy &c) {
+  DevGraph g;
+  g.n = e->n;
+  g.pool = c.pool.p;
+  g.start = c.start.p;
+  g.deg = c.deg.p;
+  g.cap = c.cap.p;
+  g.lab = c.lab.p;
+  g.lpool = c.lpool.p;
+  g.loff = c.loff.p;
+  g.nbuckets = e->nbuckets;
+  return g;
+}
+
+// Makes the engine's graph equal to `c` when it lags by exactly the last
+// update (shadow_delta_ok): only that update's touched, dirty and relabelled
+// vertices are copied (launch_sync), plus the signature rows and the apply
+// status; otherwise the whole graph (copy_graph).
+int catch_up(dsm_engine *e, const dsm_engine::GraphCopy &c) {
+  if (!kDeltaSync || !e->shadow_delta_ok || e->pool_cap != c.pool_cap || e->n >= (1u << 31))
+    return copy_graph(e, c);
+  const uint32_t nt = (uint32_t)e->last_touched.size(), nr = (uint32_t)e->last_relabel.size();
+  HIPCHK(e->d_last_touched.ensure(std::max<uint32_t>(nt, 1)));
+  HIPCHK(e->d_last_relabel.ensure(std::max<uint32_t>(nr, 1)));
+  if (nt)
+    HIPCHK(hipMemcpyAsync(e->d_last_touched.p, e->last_touched.data(), (size_t)nt * 4, hipMemcpyHostToDevice,
+                          e->stream));
+  if (nr)
+    HIPCHK(hipMemcpyAsync(e->d_last_relabel.p, e->last_relabel.data(), (size_t)nr * 4, hipMemcpyHostToDevice,
+                          e->stream));
+  // long lists queued: at most one per touched and one per dirty vertex
+  HIPCHK(e->sync_q.ensure(2 * (size_t)e->n));
+  HIPCHK(e->sync_ctl.ensure(2));
+  HIPCHK(launch_sync(graph_of(e, c), e->dev_graph(), e->d_last_touched.p, nt, e->dirty_list.p, &c.actl.p->ndirty,
+                     e->d_last_relabel.p, nr, e->sync_q.p, (uint32_t)e->sync_q.n, e->sync_ctl.p, e->stream));
+  HIPCHK(copy_buf(e->bm, c.bm, (size_t)e->nsig_rows * e->words, e->stream));
+  HIPCHK(copy_buf(e->actl, c.actl, 1, e->stream));
+  return DSM_OK;
+}
+
 int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats, bool keep = true) {
   const ApplyCtl mirror = *e->h_actl.p;
   std::swap(e->stream, e->stream2);
@@ -1460,7 +1518,7 @@ int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats, bool keep = tru
   auto run = [&]() -> int {
     HIPCHK(hipStreamWaitEvent(e->stream, e->ev_pre, 0));
     if (stats) HIPCHK(hipEventRecord(e->ev[0], e->stream));
-    if (int r = copy_graph(e, e->shadow)) return r;
+    if (int r = catch_up(e, e->shadow)) return r;
     if (int r = apply_delta_and_refresh(e, d)) return r;
     if (stats) HIPCHK(hipEventRecord(e->ev[2], e->stream));
     HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
@@ -1484,6 +1542,11 @@ int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats, bool keep = tru
   if (!keep) {
     swap_graph(e);
     *e->h_actl.p = mirror;
+  } else if (rc == DSM_OK && !e->grew) {
+    // the other copy now lags by exactly this update
+    e->shadow_delta_ok = true;
+    e->last_touched = d.tv;
+    e->last_relabel = d.rl_v;
   }
   return rc;
 }

@@ -324,6 +324,13 @@ hipError_t launch_apply_commit(const DevGraph &g, const BatchDev &b, const uint3
 hipError_t launch_apply_commit_failed(const DevGraph &g, const BatchDev &b, const uint32_t *scratch,
                                       unsigned long long pool_cap, ApplyCtl *actl, const DirtyArgs &d,
                                       hipStream_t s);
+// Copies what the last update changed from graph copy x to the stale copy y:
+// the touched vertices' slots and lists, the dirty vertices' label-sorted
+// copies (ndirty on the device), the relabelled vertices' labels. q (qcap
+// entries) and qctl[2] are scratch for the long lists.
+hipError_t launch_sync(const DevGraph &x, const DevGraph &y, const uint32_t *touched, uint32_t ntouched,
+                       const uint32_t *dirty, const uint32_t *ndirty, const uint32_t *relabeled, uint32_t nrelabeled,
+                       uint32_t *q, uint32_t qcap, uint32_t *qctl, hipStream_t s);
 hipError_t launch_relabel(const DevGraph &g, const BatchDev &b, const FilterTables &ft,
                           const DirtyArgs &d, hipStream_t s);
 hipError_t launch_filter_full(const DevGraph &g, const FilterTables &ft, hipStream_t s);

@@ -1845,6 +1845,74 @@ int max_blocks(uint32_t chunk_levels) {
   return nb > 0 ? nb : 1;
 }
 
+// ------------------------------------------------------ graph copy sync
+
+// Bringing the second copy of the graph up to date before an overlapped
+// update: instead of copying the whole graph, only what the previous update
+// changed is copied from the up-to-date copy `x` to the stale one `y`: the
+// touched vertices' slots, degrees and lists, the dirty vertices' label-sorted
+// copies and bucket offsets, and the relabelled vertices' labels. Ranges of up
+// to kSyncWave entries are copied by one wave; longer ones are queued
+// (queue_big: a block each up to kBlockList entries, the whole grid beyond)
+// for k_sync_big. A queue entry is the vertex with bit 31 set for a
+// label-sorted copy.
+constexpr uint32_t kSyncWave = 4096;
+
+__device__ __forceinline__ void sync_range(const uint32_t *from, uint32_t *to, uint32_t first, uint32_t stride,
+                                           uint32_t len) {
+  for (uint32_t j = first; j < len; j += stride) to[j] = from[j];
+}
+
+// mode 0: touched vertices (start, deg, cap and the pool slot); mode 1: dirty
+// vertices (bucket offsets and the label-sorted copy of lists longer than
+// kLabelSorted); mode 2: relabelled vertices (label).
+__global__ __launch_bounds__(256) void k_sync_small(DevGraph x, DevGraph y, const uint32_t *list, const uint32_t *count,
+                                                    uint32_t n_host, uint32_t mode, uint32_t *q, uint32_t qcap,
+                                                    uint32_t *qctl) {
+  const uint32_t lane = lane_id();
+  const uint32_t total = count ? (*count < x.n ? *count : x.n) : n_host;
+  for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < total; t += (gridDim.x * blockDim.x) >> 6) {
+    const uint32_t v = list[t];
+    if (mode == 2) {
+      if (lane == 0) y.lab[v] = x.lab[v];
+      continue;
+    }
+    const uint32_t st = x.start[v], d = x.deg[v];
+    if (mode == 0) {
+      if (lane == 0) {
+        y.start[v] = st;
+        y.deg[v] = d;
+        y.cap[v] = x.cap[v];
+      }
+    } else {
+      if (d <= kLabelSorted) continue;
+      const uint32_t nb = x.nbuckets + 1;
+      if (lane < nb) y.loff[(size_t)v * nb + lane] = x.loff[(size_t)v * nb + lane];
+    }
+    const uint32_t *from = (mode == 0 ? x.pool : x.lpool) + st;
+    uint32_t *to = (mode == 0 ? y.pool : y.lpool) + st;
+    if (d <= kSyncWave) {
+      sync_range(from, to, lane, 64, d);
+    } else if (lane == 0) {
+      queue_big(q, qcap, &qctl[0], &qctl[1], d, v | (mode << 31));
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sync_big(DevGraph x, DevGraph y, const uint32_t *q, uint32_t qcap,
+                                                  const uint32_t *qctl) {
+  for_big_lists(q, qcap, qctl[0], qctl[1], [&](uint32_t e, uint32_t first, uint32_t stride) {
+    const uint32_t v = e & 0x7FFFFFFFu;
+    const bool sorted = e >> 31;
+    const uint32_t st = x.start[v];
+    sync_range((sorted ? x.lpool : x.pool) + st, (sorted ? y.lpool : y.pool) + st, first, stride, x.deg[v]);
+  });
+}
+
+__global__ void k_sync_reset(uint32_t *qctl) {
+  if (threadIdx.x < 2) qctl[threadIdx.x] = 0;
+}
+
 }  // namespace
 
 hipError_t launch_apply_merge(const DevGraph &g, const BatchDev &b, unsigned long long scratch_cap,
@@ -1884,6 +1952,28 @@ hipError_t launch_apply_commit_failed(const DevGraph &g, const BatchDev &b, cons
                                       unsigned long long pool_cap, ApplyCtl *actl, const DirtyArgs &d,
                                       hipStream_t s) {
   return launch_apply_commit(g, b, scratch, pool_cap, actl, d, s, 1);
+}
+
+hipError_t launch_sync(const DevGraph &x, const DevGraph &y, const uint32_t *touched, uint32_t ntouched,
+                       const uint32_t *dirty, const uint32_t *ndirty, const uint32_t *relabeled, uint32_t nrelabeled,
+                       uint32_t *q, uint32_t qcap, uint32_t *qctl, hipStream_t s) {
+  hipLaunchKernelGGL(k_sync_reset, dim3(1), dim3(64), 0, s, qctl);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (ntouched) {
+    hipLaunchKernelGGL(k_sync_small, dim3((ntouched + 3) / 4 < 4096 ? (ntouched + 3) / 4 : 4096), dim3(256), 0, s, x, y,
+                       touched, (const uint32_t *)nullptr, ntouched, 0u, q, qcap, qctl);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_sync_small, dim3(4096), dim3(256), 0, s, x, y, dirty, ndirty, 0u, 1u, q, qcap, qctl);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (nrelabeled) {
+    hipLaunchKernelGGL(k_sync_small, dim3((nrelabeled + 3) / 4 < 4096 ? (nrelabeled + 3) / 4 : 4096), dim3(256), 0, s, x, y,
+                       relabeled, (const uint32_t *)nullptr, nrelabeled, 2u, q, qcap, qctl);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_sync_big, dim3(1024), dim3(256), 0, s, x, y, q, qcap, qctl);
+  return hipGetLastError();
 }
 
 hipError_t launch_relabel(const DevGraph &g, const BatchDev &b, const FilterTables &ft,

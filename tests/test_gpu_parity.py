@@ -754,3 +754,57 @@ def test_hub_copy_current_after_relabel_to_used_label(overlap):
     # 4: more changes around the now-used hubs
     batches.append(batch(fresh(0, 30, 1) + fresh(1, 30, 2) + neighbour_relabels(0, 200) + neighbour_relabels(1, 50)))
     compare_stream(n, labels, eu, ev, queries, batches, relabel=True, check_graph=True, overlap_apply=overlap)
+
+
+def test_second_copy_catch_up_paths():
+    """With overlap_apply, an update first brings the second copy of the graph
+    up to date: by copying only what the previous update changed when the copy
+    lags by exactly that update, else the whole graph. Insert-only batches
+    (applied to the current copy directly, no negatives round to hide behind),
+    mixed batches (overlapped), a hub whose list relocates, and an autotune
+    call (whose trial update on the second copy is dropped) alternate here;
+    counts, graph and bitmaps against the oracle after every batch."""
+    say("case: second-copy catch-up across direct, overlapped and dropped updates")
+    rng = np.random.default_rng(6161)
+    n = 3000
+    labels, eu, ev = random_graph(n, 15000, 3, rng)
+    queries = [qtuple("triangle", [0, 1, 2]), qtuple("tri_pendant", [1, 0, 2, 0])]
+    present = set(zip(eu.tolist(), ev.tolist()))
+    g = O.Graph(n, labels, eu, ev)
+    oq = oracle_queries(queries)
+    eng = E.Engine((n, labels, eu, ev), queries, relabel=True)
+    try:
+        for bi in range(12):
+            if bi % 3 == 1:
+                add, src, dst, lab = conflict_free_batch(n, 200, present, rng, p_add=1.0)
+            else:
+                add, src, dst, lab = conflict_free_batch(n, 300, present, rng)
+            if bi == 6:
+                # a hub: many inserts at vertex 0 relocate its list
+                batch_pairs = set(zip(np.minimum(src, dst).tolist(), np.maximum(src, dst).tolist()))
+                extra = [(1, 0, int(v), 1) for v in rng.choice(np.arange(1, n), 400, replace=False)
+                         if (0, int(v)) not in present and (0, int(v)) not in batch_pairs]
+                for _, a, b, _ in extra:
+                    present.add((a, b))
+                ea = np.array([x[0] for x in extra], np.uint8)
+                add = np.concatenate([add, ea])
+                src = np.concatenate([src, np.zeros(len(extra), np.uint32)])
+                dst = np.concatenate([dst, np.array([x[2] for x in extra], np.uint32)])
+                lab = np.concatenate([lab, np.ones(len(extra), np.uint32)])
+            want = O.run_stream(g, oq, [(add, src, dst, lab)], relabel=True)[0]
+            pos, neg = eng.process_batch(add, src, dst, lab)
+            assert pos.tolist() == want["pos"] and neg.tolist() == want["neg"], "batch %d counts" % bi
+            off, nb, dlab = eng.graph()
+            ooff, onb, olab = g.export()
+            assert off.tolist() == ooff.tolist() and np.array_equal(nb, onb), "batch %d graph" % bi
+            for qi, q in enumerate(oq):
+                cand = g.candidates(q)
+                for u in range(q.n):
+                    assert np.array_equal(eng.candidates(qi, u), cand[u].astype(bool)), \
+                        "batch %d query %d vertex %d candidates" % (bi, qi, u)
+            if bi == 8:
+                # trial rounds run beside a dropped update of the second copy
+                s_add, s_src, s_dst, s_lab = conflict_free_batch(n, 200, set(present), rng)
+                eng.autotune(s_add, s_src, s_dst, s_lab)
+    finally:
+        eng.close()
