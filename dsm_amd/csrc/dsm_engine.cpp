@@ -1295,6 +1295,7 @@ int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d) {
   e->grew = false;
   HIPCHK(launch_ctl_reset(nullptr, e->actl.p, nullptr, nullptr, 0, e->stream));
   if ((rc = do_apply_delta(e, d))) return rc;
+  HIPCHK(launch_dirty_compact(e->dirty(), e->n, e->stream));
   // the dirty list (touched vertices, relabelled ones and their neighbours) is
   // exactly the set whose label-sorted copies changed, unless the pool grew
   if (e->lpool_stale) {

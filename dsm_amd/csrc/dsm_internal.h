@@ -324,6 +324,9 @@ hipError_t launch_apply_commit(const DevGraph &g, const BatchDev &b, const uint3
 hipError_t launch_apply_commit_failed(const DevGraph &g, const BatchDev &b, const uint32_t *scratch,
                                       unsigned long long pool_cap, ApplyCtl *actl, const DirtyArgs &d,
                                       hipStream_t s);
+// The dirty list (d.list, its length added to d.actl->ndirty) from the dirty
+// bits of the update's n vertices.
+hipError_t launch_dirty_compact(const DirtyArgs &d, uint32_t n, hipStream_t s);
 // Copies what the last update changed from graph copy x to the stale copy y:
 // the touched vertices' slots and lists, the dirty vertices' label-sorted
 // copies (ndirty on the device), the relabelled vertices' labels. q (qcap

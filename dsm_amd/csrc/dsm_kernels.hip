@@ -39,16 +39,15 @@ __device__ __forceinline__ bool ge_bytes(uint64_t x, uint64_t y) {
   return (((x | H) - y) & H) == H;
 }
 
+// Marks x dirty in the update being applied: its bit only. The dirty list is
+// compacted from the bits once all marks are in (k_dirty_compact), so that
+// the marks of a hub's neighbours do not contend on a shared list counter. A
+// set bit stays set for the update: read first, so that marking an already
+// dirty vertex (the common case around hubs) costs a load, not an atomic.
 __device__ __forceinline__ void mark_dirty(const DirtyArgs &d, uint32_t x) {
   const uint32_t bit = 1u << (x & 31);
-  // a set bit stays set for the batch: read first, so that marking an already
-  // dirty vertex (the common case around hubs) costs a load, not an atomic
   if (ld_relaxed(&d.bits[x >> 5]) & bit) return;
-  const uint32_t old = atomicOr(&d.bits[x >> 5], bit);
-  if (!(old & bit)) {
-    const uint32_t p = atomicAdd(&d.actl->ndirty, 1u);
-    if (p < d.list_cap) d.list[p] = x;
-  }
+  atomicOr(&d.bits[x >> 5], bit);
 }
 
 // Only vertices with a label some query uses need their candidacy and their
@@ -61,25 +60,36 @@ __device__ __forceinline__ void mark_dirty_used(const DevGraph &g, const DirtyAr
   if (g.lab[x] != kNoLabel) mark_dirty(d, x);
 }
 
-// mark_dirty_used for one entry per lane (valid lanes only), called by every
-// lane of the wave alike: the lanes that newly set their vertex's bit take
-// their list slots with one counter add for the wave, so that the marks of a
-// hub's neighbours do not serialise on the counter.
-__device__ __forceinline__ void mark_dirty_wave(const DevGraph &g, const DirtyArgs &d, uint32_t x, bool valid) {
-  bool fresh = false;
-  if (valid && g.lab[x] != kNoLabel) {
-    const uint32_t bit = 1u << (x & 31);
-    if (!(ld_relaxed(&d.bits[x >> 5]) & bit)) fresh = !(atomicOr(&d.bits[x >> 5], bit) & bit);
-  }
-  const uint64_t m = ballot(fresh);
-  if (m == 0) return;
-  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-  uint32_t base = 0;
-  if (lane_id() == leader) base = atomicAdd(&d.actl->ndirty, (uint32_t)__popcll(m));
-  base = __shfl(base, (int)leader);
-  if (fresh) {
-    const uint32_t p = base + mbcnt(m);
-    if (p < d.list_cap) d.list[p] = x;
+// The dirty list from the dirty bits: a thread per bit word, a block prefix
+// sum of the words' counts and one counter add per block; the list comes out
+// in id order within each block's words.
+__global__ __launch_bounds__(256) void k_dirty_compact(DirtyArgs d, uint32_t nwords) {
+  __shared__ uint32_t s_sum[256];
+  __shared__ uint32_t s_base;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t w0 = blockIdx.x * 256u; w0 < nwords; w0 += gridDim.x * 256u) {
+    const uint32_t wi = w0 + tid;
+    const uint32_t bits = wi < nwords ? d.bits[wi] : 0u;
+    const uint32_t c = (uint32_t)__popc(bits);
+    s_sum[tid] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+      const uint32_t y = tid >= off ? s_sum[tid - off] : 0u;
+      __syncthreads();
+      s_sum[tid] += y;
+      __syncthreads();
+    }
+    if (tid == 255) s_base = s_sum[255] ? atomicAdd(&d.actl->ndirty, s_sum[255]) : 0u;
+    __syncthreads();
+    uint32_t p = s_base + s_sum[tid] - c;
+    uint32_t b = bits;
+    while (b) {
+      const uint32_t k = (uint32_t)__builtin_ctz(b);
+      b &= b - 1;
+      if (p < d.list_cap) d.list[p] = wi * 32u + k;
+      p++;
+    }
+    __syncthreads();
   }
 }
 
@@ -346,10 +356,7 @@ __global__ __launch_bounds__(256) void k_relabel(DevGraph g, BatchDev b, FilterT
       if (lane == 0) queue_big(b.rl_big, b.nrelabel, &d.actl->nrelabel_big, &d.actl->nrelabel_huge, na, x);
       continue;
     }
-    for (uint32_t j0 = 0; j0 < na; j0 += 64) {
-      const uint32_t j = j0 + lane;
-      mark_dirty_wave(g, d, j < na ? A[j] : 0u, j < na);
-    }
+    for (uint32_t j = lane; j < na; j += 64) mark_dirty_used(g, d, A[j]);
   }
 }
 
@@ -360,11 +367,7 @@ __global__ __launch_bounds__(256) void k_relabel_big(DevGraph g, BatchDev b, Dir
                                                                                          uint32_t stride) {
     const uint32_t na = g.deg[x];
     const uint32_t *A = g.pool + g.start[x];
-    // first - lane_id() is the wave's start, so its lanes run the loop alike
-    for (uint32_t j0 = first - lane_id(); j0 < na; j0 += stride) {
-      const uint32_t j = j0 + lane_id();
-      mark_dirty_wave(g, d, j < na ? A[j] : 0u, j < na);
-    }
+    for (uint32_t j = first; j < na; j += stride) mark_dirty_used(g, d, A[j]);
   });
 }
 
@@ -1952,6 +1955,13 @@ hipError_t launch_apply_commit_failed(const DevGraph &g, const BatchDev &b, cons
                                       unsigned long long pool_cap, ApplyCtl *actl, const DirtyArgs &d,
                                       hipStream_t s) {
   return launch_apply_commit(g, b, scratch, pool_cap, actl, d, s, 1);
+}
+
+hipError_t launch_dirty_compact(const DirtyArgs &d, uint32_t n, hipStream_t s) {
+  const uint32_t nwords = (n + 31) / 32;
+  const uint32_t blocks = (nwords + 255) / 256 < 4096 ? (nwords + 255) / 256 : 4096;
+  hipLaunchKernelGGL(k_dirty_compact, dim3(blocks ? blocks : 1), dim3(256), 0, s, d, nwords);
+  return hipGetLastError();
 }
 
 hipError_t launch_sync(const DevGraph &x, const DevGraph &y, const uint32_t *touched, uint32_t ntouched,
