@@ -460,15 +460,24 @@ __global__ __launch_bounds__(256) void k_filter_light(DevGraph g, FilterTables f
     const uint32_t v = d.list[i];
     atomicAnd(&d.bits[v >> 5], ~(1u << (v & 31)));
     const uint32_t L = g.lab[v];
-    if (L == kNoLabel || ft.sig_lo[L] == ft.sig_hi[L]) continue;
-    const uint32_t dv = g.deg[v];
-    entries += dv;
-    bits += ft.sig_hi[L] - ft.sig_lo[L];
-    if (dv > (uint32_t)kLight) {
-      const uint32_t p = atomicAdd(&d.actl->nheavy, 1u);
-      if (p < d.list_cap) d.heavy[p] = v;
-      continue;
+    const bool has = L != kNoLabel && ft.sig_lo[L] != ft.sig_hi[L];
+    const uint32_t dv = has ? g.deg[v] : 0u;
+    if (has) {
+      entries += dv;
+      bits += ft.sig_hi[L] - ft.sig_lo[L];
     }
+    // long lists go to the heavy list, one counter add per wave
+    const bool heavy = has && dv > (uint32_t)kLight;
+    const uint64_t hm = ballot(heavy);
+    if (hm) {
+      const uint32_t leader = (uint32_t)__builtin_ctzll(hm);
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&d.actl->nheavy, (uint32_t)__popcll(hm));
+      base = __shfl(base, (int)leader);
+      const uint32_t p = base + mbcnt(hm);
+      if (heavy && p < d.list_cap) d.heavy[p] = v;
+    }
+    if (!has || heavy) continue;
     for (int l = 0; l < kMaxRel; l++) cnt[l] = 0;
     const uint32_t *A = g.pool + g.start[v];
     for (uint32_t j = 0; j < dv; j++) {
