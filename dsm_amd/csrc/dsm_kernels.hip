@@ -215,6 +215,10 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
 // them), with the prefix counts k_apply_merge stored.
 __global__ __launch_bounds__(256) void k_apply_merge_big(DevGraph g, BatchDev b, uint32_t *scratch,
                                                          const ApplyCtl *actl) {
+  // the list's inserts and deletes with their prefix counts, staged in LDS:
+  // every chunk of the list searches them
+  __shared__ uint32_t s_I[kMaxVertexChanges], s_D[kMaxVertexChanges];
+  __shared__ uint32_t s_icum[kMaxVertexChanges + 1], s_dcum[kMaxVertexChanges + 1];
   const uint32_t lane = lane_id();
   for_big_lists(b.big_merge, b.ntouched, actl->nmerge_big, actl->nmerge_huge, [&](uint32_t t, uint32_t tid,
                                                                                    uint32_t nth) {
@@ -225,10 +229,20 @@ __global__ __launch_bounds__(256) void k_apply_merge_big(DevGraph g, BatchDev b,
     const uint32_t *A = g.pool + g.start[x];
     const uint32_t i0 = b.ioff[t], ni = b.ioff[t + 1] - i0;
     const uint32_t d0 = b.doff[t], nd = b.doff[t + 1] - d0;
-    const uint32_t *I = b.ins + i0;
-    const uint32_t *D = b.del + d0;
-    const uint32_t *icum = b.gicum + i0 + t;
-    const uint32_t *dcum = b.gdcum + d0 + t;
+    __syncthreads();  // the previous list's stage is no longer read
+    for (uint32_t j = threadIdx.x; j <= ni; j += blockDim.x) {
+      if (j < ni) s_I[j] = b.ins[i0 + j];
+      s_icum[j] = b.gicum[i0 + t + j];
+    }
+    for (uint32_t j = threadIdx.x; j <= nd; j += blockDim.x) {
+      if (j < nd) s_D[j] = b.del[d0 + j];
+      s_dcum[j] = b.gdcum[d0 + t + j];
+    }
+    __syncthreads();
+    const uint32_t *I = s_I;
+    const uint32_t *D = s_D;
+    const uint32_t *icum = s_icum;
+    const uint32_t *dcum = s_dcum;
     uint32_t *out = scratch + b.soff[t];
     // waves take 64-entry chunks of A in turn; the deletes and inserts that
     // fall inside a chunk's key range are found once per chunk, so each entry
