@@ -127,6 +127,10 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
                                                                   uint32_t *scratch, ApplyCtl *actl) {
   __shared__ uint16_t s_icum[kApplyWaves][kMaxVertexChanges + 1];
   __shared__ uint16_t s_dcum[kApplyWaves][kMaxVertexChanges + 1];
+  // up to kStageChanges inserts and deletes of the wave's vertex, staged in
+  // LDS for the per-entry searches (more are searched in global memory)
+  constexpr uint32_t kStageChanges = 128;
+  __shared__ uint32_t s_I[kApplyWaves][kStageChanges], s_D[kApplyWaves][kStageChanges];
   const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
   uint16_t *icum = s_icum[wv];
   uint16_t *dcum = s_dcum[wv];
@@ -138,6 +142,14 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_merge(DevGraph g, Ba
     const uint32_t d0 = b.doff[t], nd = b.doff[t + 1] - d0;
     const uint32_t *I = b.ins + i0;
     const uint32_t *D = b.del + d0;
+    wave_sync();  // the previous vertex's stage is no longer read
+    if (ni <= kStageChanges && nd <= kStageChanges) {
+      for (uint32_t j = lane; j < ni; j += 64) s_I[wv][j] = I[j];
+      for (uint32_t j = lane; j < nd; j += 64) s_D[wv][j] = D[j];
+      wave_sync();
+      I = s_I[wv];
+      D = s_D[wv];
+    }
     // exclusive prefix of "insert is new" and "delete hits"
     uint32_t run = 0;
     for (uint32_t c = 0; c < ni; c += 64) {
