@@ -584,14 +584,30 @@ __device__ __forceinline__ void bucket_count(const DevGraph &g, const uint8_t *s
                                              uint32_t lo, uint32_t hi, uint32_t bits, uint32_t *cnt,
                                              const uint8_t *bk_in, uint8_t *bk_out) {
   const uint32_t lane = lane_id();
-  for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
-    const uint32_t j = c0 + lane;
-    const bool valid = j < hi;
-    const uint32_t b = valid ? (bk_in ? (uint32_t)bk_in[j] : bucket_of(g, s_lmap, A[j])) : 0u;
-    if (valid && bk_out) bk_out[j] = (uint8_t)b;
-    const uint64_t pm = bucket_peers(b, valid, bits);
-    if (valid && mbcnt(pm) == 0) cnt[b] += (uint32_t)__popcll(pm);
-    wave_sync();
+  // four chunks at a time: their entries, then their label reads, are all in
+  // flight before the first chunk's counts are taken
+  constexpr uint32_t kU = 4;
+  for (uint32_t c0 = lo; c0 < hi; c0 += 64 * kU) {
+    uint32_t x[kU], b[kU];
+#pragma unroll
+    for (uint32_t u = 0; u < kU; u++) {
+      const uint32_t j = c0 + u * 64 + lane;
+      x[u] = j < hi && !bk_in ? A[j] : 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kU; u++) {
+      const uint32_t j = c0 + u * 64 + lane;
+      b[u] = j < hi ? (bk_in ? (uint32_t)bk_in[j] : bucket_of(g, s_lmap, x[u])) : 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kU; u++) {
+      const uint32_t j = c0 + u * 64 + lane;
+      const bool valid = j < hi;
+      if (valid && bk_out) bk_out[j] = (uint8_t)b[u];
+      const uint64_t pm = bucket_peers(b[u], valid, bits);
+      if (valid && mbcnt(pm) == 0) cnt[b[u]] += (uint32_t)__popcll(pm);
+      wave_sync();
+    }
   }
 }
 
