@@ -393,7 +393,16 @@ __global__ __launch_bounds__(256) void k_relabel_big(DevGraph g, BatchDev b, Dir
                                                                                          uint32_t stride) {
     const uint32_t na = g.deg[x];
     const uint32_t *A = g.pool + g.start[x];
-    for (uint32_t j = first; j < na; j += stride) mark_dirty_used(g, d, A[j]);
+    // four entries per thread in flight before their marks
+    uint32_t j = first;
+    for (; j + 3 * stride < na; j += 4 * stride) {
+      const uint32_t x0 = A[j], x1 = A[j + stride], x2 = A[j + 2 * stride], x3 = A[j + 3 * stride];
+      mark_dirty_used(g, d, x0);
+      mark_dirty_used(g, d, x1);
+      mark_dirty_used(g, d, x2);
+      mark_dirty_used(g, d, x3);
+    }
+    for (; j < na; j += stride) mark_dirty_used(g, d, A[j]);
   });
 }
 
