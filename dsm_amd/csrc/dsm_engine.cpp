@@ -2202,14 +2202,15 @@ void publish(dsm_server *sv, ServerReq *r, int status) {
 
 // One DFS round over the current graph: the positives of P (if any) and the
 // negatives of R (if any); then P's count slot is copied to the host. Waits
-// for the round; while it runs, computes R's delta into *d. With `stats`, the
-// round's work counters and time go to the engine's statistics.
+// for the round; while it runs, computes R's delta into *d, unless
+// `have_delta` says that *d already holds it. With `stats`, the round's work
+// counters and time go to the engine's statistics.
 // One pipelined round: the positives of P and the negatives of R in one DFS
 // launch on the current graph. With `applied` given and overlap_apply on, R's
 // update is applied to the second copy of the graph meanwhile and *applied is
 // set; otherwise the caller applies it after the round.
 int pipe_round(dsm_engine *e, PinnedBuf<uint32_t> &h_rec, DevBuf<uint32_t> &d_rec, PipeItem *P, PipeItem *R,
-               HostDelta *d, bool stats, bool *applied) {
+               HostDelta *d, bool stats, bool *applied, bool have_delta = false) {
   const uint32_t nq = (uint32_t)e->queries.size();
   int rc;
   const uint32_t np = P ? (uint32_t)P->ps.size() : 0u, nn = R ? (uint32_t)R->ns.size() : 0u;
@@ -2243,7 +2244,7 @@ int pipe_round(dsm_engine *e, PinnedBuf<uint32_t> &h_rec, DevBuf<uint32_t> &d_re
   if (P)
     HIPCHK(hipMemcpyAsync(e->h_counts.p + (size_t)P->slot * nq * 2, e->counts.p + (size_t)P->slot * nq * 2,
                           (size_t)nq * 2 * 8, hipMemcpyDeviceToHost, e->stream));
-  if (R && d) host_delta(e, R->rec, R->nrec, *d);
+  if (R && d && !have_delta) host_delta(e, R->rec, R->nrec, *d);
   if (overlap) {
     if ((rc = overlap_apply(e, *d, stats))) return rc;
     *applied = true;
@@ -2462,13 +2463,53 @@ int dsm_process_batches(dsm_engine *e, const dsm_change *changes, const uint32_t
     apply_timed = false;
     return DSM_OK;
   };
+  // A batch's host delta (its records as per-vertex insert and delete lists)
+  // depends on its records alone. A worker thread computes them in batch
+  // order, ahead of the rounds, while this thread waits for the GPU; a round
+  // then enqueues its update right behind its DFS launch instead of after
+  // sorting its batch (at 32,768 records, a few ms per batch).
+  std::vector<HostDelta> deltas(num_batches);
+  std::mutex dmu;
+  std::condition_variable dcv;
+  uint32_t dready = 0;
+  bool dstop = false;
+  std::thread dworker([&] {
+    for (uint32_t b = 0; b < num_batches; b++) {
+      {
+        std::lock_guard<std::mutex> lk(dmu);
+        if (dstop) return;
+      }
+      host_delta(e, items[b].rec, items[b].nrec, deltas[b]);
+      {
+        std::lock_guard<std::mutex> lk(dmu);
+        dready = b + 1;
+      }
+      dcv.notify_one();
+    }
+  });
+  struct JoinWorker {
+    std::thread &t;
+    std::mutex &mu;
+    bool &stop;
+    ~JoinWorker() {
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;
+      }
+      t.join();
+    }
+  } join_worker{dworker, dmu, dstop};
   PipeItem *P = nullptr;
   for (uint32_t b = 0; b < num_batches; b++) {
     PipeItem *R = &items[b];
     R->slot = P ? P->slot ^ 1u : 0u;
-    HostDelta d;
+    {
+      std::unique_lock<std::mutex> lk(dmu);
+      dcv.wait(lk, [&] { return dready > b; });
+    }
+    HostDelta d = std::move(deltas[b]);
     bool applied = false;
-    if ((rc = pipe_round(e, e->pipe_h_rec, e->pipe_d_rec, P, R, &d, true, &applied))) return rc;
+    if ((rc = pipe_round(e, e->pipe_h_rec, e->pipe_d_rec, P, R, &d, true, &applied, true))) return rc;
     if ((rc = take_apply_time())) return rc;
     if (P) publish_item(b - 1);
     if (!applied) {
