@@ -2467,16 +2467,20 @@ int dsm_process_batches(dsm_engine *e, const dsm_change *changes, const uint32_t
   // depends on its records alone. A worker thread computes them in batch
   // order, ahead of the rounds, while this thread waits for the GPU; a round
   // then enqueues its update right behind its DFS launch instead of after
-  // sorting its batch (at 32,768 records, a few ms per batch).
+  // sorting its batch (at 32,768 records, a few ms per batch). The worker
+  // stays at most kDeltasAhead batches ahead of the rounds, so the deltas
+  // held at once stay bounded however many batches the call takes.
+  constexpr uint32_t kDeltasAhead = 2;
   std::vector<HostDelta> deltas(num_batches);
   std::mutex dmu;
-  std::condition_variable dcv;
-  uint32_t dready = 0;
+  std::condition_variable ready_cv, taken_cv;
+  uint32_t dready = 0, dtaken = 0;
   bool dstop = false;
   std::thread dworker([&] {
     for (uint32_t b = 0; b < num_batches; b++) {
       {
-        std::lock_guard<std::mutex> lk(dmu);
+        std::unique_lock<std::mutex> lk(dmu);
+        taken_cv.wait(lk, [&] { return dstop || b < dtaken + kDeltasAhead; });
         if (dstop) return;
       }
       host_delta(e, items[b].rec, items[b].nrec, deltas[b]);
@@ -2484,29 +2488,33 @@ int dsm_process_batches(dsm_engine *e, const dsm_change *changes, const uint32_t
         std::lock_guard<std::mutex> lk(dmu);
         dready = b + 1;
       }
-      dcv.notify_one();
+      ready_cv.notify_one();
     }
   });
   struct JoinWorker {
     std::thread &t;
     std::mutex &mu;
+    std::condition_variable &cv;
     bool &stop;
     ~JoinWorker() {
       {
         std::lock_guard<std::mutex> lk(mu);
         stop = true;
       }
+      cv.notify_one();
       t.join();
     }
-  } join_worker{dworker, dmu, dstop};
+  } join_worker{dworker, dmu, taken_cv, dstop};
   PipeItem *P = nullptr;
   for (uint32_t b = 0; b < num_batches; b++) {
     PipeItem *R = &items[b];
     R->slot = P ? P->slot ^ 1u : 0u;
     {
       std::unique_lock<std::mutex> lk(dmu);
-      dcv.wait(lk, [&] { return dready > b; });
+      ready_cv.wait(lk, [&] { return dready > b; });
+      dtaken = b + 1;
     }
+    taken_cv.notify_one();
     HostDelta d = std::move(deltas[b]);
     bool applied = false;
     if ((rc = pipe_round(e, e->pipe_h_rec, e->pipe_d_rec, P, R, &d, true, &applied, true))) return rc;
