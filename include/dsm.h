@@ -212,8 +212,10 @@ int dsm_process_batch(dsm_engine *e, const dsm_change *changes, uint32_t num_cha
 /* A sequence of batches, each with the semantics of dsm_process_batch and in
  * order, pipelined: each DFS launch matches the positives of batch b together
  * with the negatives of batch b+1, which are evaluated on the same graph
- * state, and batch b+1's host-side delta is computed while that launch runs.
- * Batch b is changes[batch_offsets[b] .. batch_offsets[b+1]); the counts go
+ * state. The batches' host-side deltas are computed by a worker thread that
+ * the call starts and joins, at most two batches ahead of the launches, while
+ * the calling thread waits for the GPU; the records must stay unchanged until
+ * the call returns. Batch b is changes[batch_offsets[b] .. batch_offsets[b+1]); the counts go
  * to pos_counts / neg_counts [num_batches][num_queries]; done_s (optional)
  * receives the steady-clock second at which each batch's counts were ready.
  * With overlap_apply, batch b+1's update is applied to the second copy of
