@@ -621,26 +621,52 @@ __device__ __forceinline__ void bucket_count(const DevGraph &g, const uint8_t *s
 }
 
 // Scatters A[lo, hi) into out by bucket, cnt[b] holding the next position of
-// bucket b; id order is kept inside each bucket.
+// bucket b; id order is kept inside each bucket. Four chunks' entries and
+// buckets are loaded before the first chunk is placed.
 __device__ __forceinline__ void bucket_scatter(const DevGraph &g, const uint8_t *s_lmap, const uint32_t *A,
                                                uint32_t lo, uint32_t hi, uint32_t bits, uint32_t *cnt,
                                                uint32_t *out, const uint8_t *bk_in) {
   const uint32_t lane = lane_id();
-  for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
-    const uint32_t j = c0 + lane;
-    const bool valid = j < hi;
-    const uint32_t x = valid ? A[j] : 0u;
-    const uint32_t b = valid ? (bk_in ? (uint32_t)bk_in[j] : bucket_of(g, s_lmap, x)) : 0u;
-    const uint64_t pm = bucket_peers(b, valid, bits);
-    const uint32_t rank = mbcnt(pm);
-    const uint32_t base = valid ? cnt[b] : 0u;
-    wave_sync();
-    if (valid) {
-      out[base + rank] = x;
-      if (rank == 0) cnt[b] = base + (uint32_t)__popcll(pm);
+  constexpr uint32_t kU = 4;
+  for (uint32_t c0 = lo; c0 < hi; c0 += 64 * kU) {
+    uint32_t x[kU], b[kU];
+#pragma unroll
+    for (uint32_t u = 0; u < kU; u++) {
+      const uint32_t j = c0 + u * 64 + lane;
+      x[u] = j < hi ? A[j] : 0u;
     }
-    wave_sync();
+#pragma unroll
+    for (uint32_t u = 0; u < kU; u++) {
+      const uint32_t j = c0 + u * 64 + lane;
+      b[u] = j < hi ? (bk_in ? (uint32_t)bk_in[j] : bucket_of(g, s_lmap, x[u])) : 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kU; u++) {
+      const bool valid = c0 + u * 64 + lane < hi;
+      const uint64_t pm = bucket_peers(b[u], valid, bits);
+      const uint32_t rank = mbcnt(pm);
+      const uint32_t base = valid ? cnt[b[u]] : 0u;
+      wave_sync();
+      if (valid) {
+        out[base + rank] = x[u];
+        if (rank == 0) cnt[b[u]] = base + (uint32_t)__popcll(pm);
+      }
+      wave_sync();
+    }
   }
+}
+
+// Exclusive prefix sum over the lanes (a wave-wide scan of shuffles); the
+// inclusive total comes back in *total.
+__device__ __forceinline__ uint32_t wave_excl_sum(uint32_t c, uint32_t *total) {
+  const uint32_t lane = lane_id();
+  uint32_t incl = c;
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off);
+    if (lane >= off) incl += y;
+  }
+  *total = __builtin_amdgcn_readlane(incl, 63);
+  return incl - c;
 }
 
 constexpr uint32_t kLbuildBlock = 4096;  // lists longer than this are sorted by a whole block
@@ -658,16 +684,15 @@ __device__ __forceinline__ void build_one(const DevGraph &g, const uint8_t *s_lm
   wave_sync();
   uint8_t *bk = bucket_bytes ? bucket_bytes + g.start[v] : nullptr;
   bucket_count(g, s_lmap, A, 0, d, bits, cnt, nullptr, bk);
-  if (lane == 0) {
-    uint32_t run = 0;
-    for (uint32_t k = 0; k < nb; k++) {
-      const uint32_t c = cnt[k];
-      cnt[k] = run;
-      row[k] = run;
-      run += c;
-    }
-    row[nb] = run;
+  // bucket starts: an exclusive scan with a lane per bucket (nb <= kMaxRel + 1 < 64)
+  uint32_t total;
+  const uint32_t c = lane < nb ? cnt[lane] : 0u;
+  const uint32_t first = wave_excl_sum(c, &total);
+  if (lane < nb) {
+    cnt[lane] = first;
+    row[lane] = first;
   }
+  if (lane == 0) row[nb] = total;
   wave_sync();
   bucket_scatter(g, s_lmap, A, 0, d, bits, cnt, g.lpool + g.start[v], bk);
 }
@@ -736,7 +761,6 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_big(DevGraph g, co
                                                                   HugeSort hs) {
   __shared__ uint8_t s_lmap[256];
   __shared__ uint32_t s_cnt[kLbuildWaves][kMaxRel + 2];
-  __shared__ uint32_t s_base[kMaxRel + 2];
   __shared__ uint32_t s_queued;
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_lmap[i] = lmap[i];
   __syncthreads();
@@ -786,30 +810,24 @@ __global__ __launch_bounds__(kLbuildWaves * 64) void k_lbuild_big(DevGraph g, co
     wave_sync();
     bucket_count(g, s_lmap, A, lo, hi, bits, cnt, nullptr, bk);
     __syncthreads();
-    if (threadIdx.x < nb) {
+    if (wv == 0) {
+      // wave 0, a lane per bucket (nb <= kMaxRel + 1 < 64): the bucket's total
+      // over the waves, the bucket starts by a scan over the lanes, then each
+      // wave's start inside the bucket
       uint32_t tot = 0;
-      for (uint32_t w = 0; w < kLbuildWaves; w++) tot += s_cnt[w][threadIdx.x];
-      s_base[threadIdx.x] = tot;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t run = 0;
-      for (uint32_t k = 0; k < nb; k++) {
-        const uint32_t c = s_base[k];
-        s_base[k] = run;
-        row[k] = run;
-        run += c;
+      if (lane < nb)
+        for (uint32_t w = 0; w < kLbuildWaves; w++) tot += s_cnt[w][lane];
+      uint32_t total;
+      uint32_t run = wave_excl_sum(tot, &total);
+      if (lane < nb) {
+        row[lane] = run;
+        for (uint32_t w = 0; w < kLbuildWaves; w++) {
+          const uint32_t c = s_cnt[w][lane];
+          s_cnt[w][lane] = run;
+          run += c;
+        }
       }
-      row[nb] = run;
-    }
-    __syncthreads();
-    if (threadIdx.x < nb) {
-      uint32_t run = s_base[threadIdx.x];
-      for (uint32_t w = 0; w < kLbuildWaves; w++) {
-        const uint32_t c = s_cnt[w][threadIdx.x];
-        s_cnt[w][threadIdx.x] = run;
-        run += c;
-      }
+      if (lane == 0) row[nb] = total;
     }
     __syncthreads();
     bucket_scatter(g, s_lmap, A, lo, hi, bits, cnt, g.lpool + g.start[v], bk);
