@@ -1939,9 +1939,35 @@ int max_blocks(uint32_t chunk_levels) {
 // label-sorted copy.
 constexpr uint32_t kSyncWave = 4096;
 
-__device__ __forceinline__ void sync_range(const uint32_t *from, uint32_t *to, uint32_t first, uint32_t stride,
-                                           uint32_t len) {
-  for (uint32_t j = first; j < len; j += stride) to[j] = from[j];
+// Copies from[0, len) to to[0, len); the caller's threads are first, first +
+// stride, ... (stride >= 4). Both ranges start at the same offset of a pool
+// (or label-sorted pool) of either copy, so they share their alignment modulo
+// 16 bytes: the bulk moves in 16-byte pieces, four in flight per thread, with
+// fewer than four entries copied singly at each end. Ranges whose alignments
+// differ are copied entry by entry.
+__device__ __forceinline__ void sync_range(const uint32_t *__restrict__ from, uint32_t *__restrict__ to,
+                                           uint32_t first, uint32_t stride, uint32_t len) {
+  if ((((uintptr_t)from ^ (uintptr_t)to) & 15u) != 0) {
+    for (uint32_t j = first; j < len; j += stride) to[j] = from[j];
+    return;
+  }
+  const uint32_t lead = (uint32_t)((16u - ((uintptr_t)from & 15u)) & 15u) >> 2;
+  const uint32_t head = lead < len ? lead : len;
+  if (first < head) to[first] = from[first];
+  const uint32_t nv = (len - head) >> 2;
+  const uint4 *__restrict__ fv = reinterpret_cast<const uint4 *>(from + head);
+  uint4 *__restrict__ tv = reinterpret_cast<uint4 *>(to + head);
+  uint32_t j = first;
+  for (; j + 3 * stride < nv; j += 4 * stride) {
+    const uint4 a = fv[j], b = fv[j + stride], c = fv[j + 2 * stride], e = fv[j + 3 * stride];
+    tv[j] = a;
+    tv[j + stride] = b;
+    tv[j + 2 * stride] = c;
+    tv[j + 3 * stride] = e;
+  }
+  for (; j < nv; j += stride) tv[j] = fv[j];
+  const uint32_t done = head + 4 * nv;
+  if (first < len - done) to[done + first] = from[done + first];
 }
 
 // mode 0: touched vertices (start, deg, cap and the pool slot); mode 1: dirty
