@@ -119,6 +119,45 @@ __device__ __forceinline__ void queue_big(uint32_t *arr, uint32_t cap, uint32_t 
   else arr[atomicAdd(nblock, 1u)] = v;
 }
 
+// Copies from[0, len) to to[0, len) (disjoint); the caller's threads are
+// first, first + stride, ... (stride >= 4). When both ranges share their
+// alignment modulo 16 bytes (the same offset of a pool in either copy of the
+// graph), the bulk moves in 16-byte pieces, four in flight per thread, with
+// fewer than four entries copied singly at each end; otherwise entry by
+// entry, four in flight per thread.
+__device__ __forceinline__ void copy_range(const uint32_t *__restrict__ from, uint32_t *__restrict__ to,
+                                           uint32_t first, uint32_t stride, uint32_t len) {
+  if ((((uintptr_t)from ^ (uintptr_t)to) & 15u) != 0) {
+    uint32_t j = first;
+    for (; j + 3 * stride < len; j += 4 * stride) {
+      const uint32_t a = from[j], b = from[j + stride], c = from[j + 2 * stride], e = from[j + 3 * stride];
+      to[j] = a;
+      to[j + stride] = b;
+      to[j + 2 * stride] = c;
+      to[j + 3 * stride] = e;
+    }
+    for (; j < len; j += stride) to[j] = from[j];
+    return;
+  }
+  const uint32_t lead = (uint32_t)((16u - ((uintptr_t)from & 15u)) & 15u) >> 2;
+  const uint32_t head = lead < len ? lead : len;
+  if (first < head) to[first] = from[first];
+  const uint32_t nv = (len - head) >> 2;
+  const uint4 *__restrict__ fv = reinterpret_cast<const uint4 *>(from + head);
+  uint4 *__restrict__ tv = reinterpret_cast<uint4 *>(to + head);
+  uint32_t j = first;
+  for (; j + 3 * stride < nv; j += 4 * stride) {
+    const uint4 a = fv[j], b = fv[j + stride], c = fv[j + 2 * stride], e = fv[j + 3 * stride];
+    tv[j] = a;
+    tv[j + stride] = b;
+    tv[j + 2 * stride] = c;
+    tv[j + 3 * stride] = e;
+  }
+  for (; j < nv; j += stride) tv[j] = fv[j];
+  const uint32_t done = head + 4 * nv;
+  if (first < len - done) to[done + first] = from[done + first];
+}
+
 // One wave per touched vertex: new list = (A \ D) u I written to scratch.
 // I and D are sorted and hold at most kMaxVertexChanges entries each; every
 // key has a single final operation, so I and D are disjoint.
@@ -324,7 +363,7 @@ __global__ __launch_bounds__(kApplyWaves * 64) void k_apply_commit(DevGraph g, B
       // k_apply_commit_big copies; the new degree is only read by later kernels
       if (lane == 0) queue_big(b.big_commit, b.ntouched, &actl->ncommit_big, &actl->ncommit_huge, nd, t);
     } else {
-      for (uint32_t j = lane; j < nd; j += 64) g.pool[dst + j] = scratch[so + j];
+      copy_range(scratch + so, g.pool + dst, lane, 64, nd);
     }
     if (lane == 0) {
       g.deg[x] = nd;
@@ -343,7 +382,7 @@ __global__ __launch_bounds__(256) void k_apply_commit_big(DevGraph g, BatchDev b
     const uint32_t nd = b.ndeg[t];
     const uint32_t dst = g.start[b.tv[t]];
     const unsigned long long so = b.soff[t];
-    for (uint32_t j = first; j < nd; j += stride) g.pool[dst + j] = scratch[so + j];
+    copy_range(scratch + so, g.pool + dst, first, stride, nd);
   });
 }
 
@@ -515,7 +554,17 @@ __global__ __launch_bounds__(256) void k_filter_light(DevGraph g, FilterTables f
     if (!has || heavy) continue;
     for (int l = 0; l < kMaxRel; l++) cnt[l] = 0;
     const uint32_t *A = g.pool + g.start[v];
-    for (uint32_t j = 0; j < dv; j++) {
+    // four entries, then their four label reads, in flight before the counts
+    uint32_t j = 0;
+    for (; j + 4 <= dv; j += 4) {
+      const uint32_t x0 = A[j], x1 = A[j + 1], x2 = A[j + 2], x3 = A[j + 3];
+      const uint32_t l0 = g.lab[x0], l1 = g.lab[x1], l2 = g.lab[x2], l3 = g.lab[x3];
+      const uint32_t li[4] = {s_lmap[l0], s_lmap[l1], s_lmap[l2], s_lmap[l3]};
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (li[u] != 0xFF && cnt[li[u]] < 15) cnt[li[u]]++;
+    }
+    for (; j < dv; j++) {
       const uint32_t li = s_lmap[g.lab[A[j]]];
       if (li != 0xFF && cnt[li] < 15) cnt[li]++;
     }
@@ -1939,37 +1988,6 @@ int max_blocks(uint32_t chunk_levels) {
 // label-sorted copy.
 constexpr uint32_t kSyncWave = 4096;
 
-// Copies from[0, len) to to[0, len); the caller's threads are first, first +
-// stride, ... (stride >= 4). Both ranges start at the same offset of a pool
-// (or label-sorted pool) of either copy, so they share their alignment modulo
-// 16 bytes: the bulk moves in 16-byte pieces, four in flight per thread, with
-// fewer than four entries copied singly at each end. Ranges whose alignments
-// differ are copied entry by entry.
-__device__ __forceinline__ void sync_range(const uint32_t *__restrict__ from, uint32_t *__restrict__ to,
-                                           uint32_t first, uint32_t stride, uint32_t len) {
-  if ((((uintptr_t)from ^ (uintptr_t)to) & 15u) != 0) {
-    for (uint32_t j = first; j < len; j += stride) to[j] = from[j];
-    return;
-  }
-  const uint32_t lead = (uint32_t)((16u - ((uintptr_t)from & 15u)) & 15u) >> 2;
-  const uint32_t head = lead < len ? lead : len;
-  if (first < head) to[first] = from[first];
-  const uint32_t nv = (len - head) >> 2;
-  const uint4 *__restrict__ fv = reinterpret_cast<const uint4 *>(from + head);
-  uint4 *__restrict__ tv = reinterpret_cast<uint4 *>(to + head);
-  uint32_t j = first;
-  for (; j + 3 * stride < nv; j += 4 * stride) {
-    const uint4 a = fv[j], b = fv[j + stride], c = fv[j + 2 * stride], e = fv[j + 3 * stride];
-    tv[j] = a;
-    tv[j + stride] = b;
-    tv[j + 2 * stride] = c;
-    tv[j + 3 * stride] = e;
-  }
-  for (; j < nv; j += stride) tv[j] = fv[j];
-  const uint32_t done = head + 4 * nv;
-  if (first < len - done) to[done + first] = from[done + first];
-}
-
 // mode 0: touched vertices (start, deg, cap and the pool slot); mode 1: dirty
 // vertices (bucket offsets and the label-sorted copy of lists longer than
 // kLabelSorted); mode 2: relabelled vertices (label).
@@ -1999,7 +2017,7 @@ __global__ __launch_bounds__(256) void k_sync_small(DevGraph x, DevGraph y, cons
     const uint32_t *from = (mode == 0 ? x.pool : x.lpool) + st;
     uint32_t *to = (mode == 0 ? y.pool : y.lpool) + st;
     if (d <= kSyncWave) {
-      sync_range(from, to, lane, 64, d);
+      copy_range(from, to, lane, 64, d);
     } else if (lane == 0) {
       queue_big(q, qcap, &qctl[0], &qctl[1], d, v | (mode << 31));
     }
@@ -2012,7 +2030,7 @@ __global__ __launch_bounds__(256) void k_sync_big(DevGraph x, DevGraph y, const 
     const uint32_t v = e & 0x7FFFFFFFu;
     const bool sorted = e >> 31;
     const uint32_t st = x.start[v];
-    sync_range((sorted ? x.lpool : x.pool) + st, (sorted ? y.lpool : y.pool) + st, first, stride, x.deg[v]);
+    copy_range((sorted ? x.lpool : x.pool) + st, (sorted ? y.lpool : y.pool) + st, first, stride, x.deg[v]);
   });
 }
 
