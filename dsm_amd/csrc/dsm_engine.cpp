@@ -1289,12 +1289,28 @@ void take_refine_time(dsm_engine *e) {
 // dirty vertices' label-sorted copies, then refilter the dirty vertices (long
 // lists take their neighbour-label counts from those copies). No wait at the
 // end.
-int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d) {
+// The label-sorted copies that a catch-up leaves to the update that follows
+// it: those of the last update's dirty vertices (the shared dirty list still
+// holds them until the next compaction), copied from `from` once the new
+// update has marked its own dirty vertices, skipping those, whose copies it
+// rebuilds from their lists anyway.
+struct SortedSync {
+  DevGraph from;
+  const uint32_t *ndirty;
+};
+
+int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d, const SortedSync *sorted_sync = nullptr) {
   int rc;
   e->shadow_delta_ok = false;
   e->grew = false;
   HIPCHK(launch_ctl_reset(nullptr, e->actl.p, nullptr, nullptr, 0, e->stream));
   if ((rc = do_apply_delta(e, d))) return rc;
+  // the catch-up's deferred label-sorted copies: this update's dirty bits are
+  // all set and the dirty list still holds the last update's vertices. After
+  // a pool growth the slots moved and every copy is rebuilt instead.
+  if (sorted_sync && !e->grew && !e->lpool_stale)
+    HIPCHK(launch_sync_sorted(sorted_sync->from, e->dev_graph(), e->dirty_list.p, sorted_sync->ndirty,
+                              e->dirty_bits.p, e->sync_q.p, (uint32_t)e->sync_q.n, e->sync_ctl.p, e->stream));
   HIPCHK(launch_dirty_compact(e->dirty(), e->n, e->stream));
   // the dirty list (touched vertices, relabelled ones and their neighbours) is
   // exactly the set whose label-sorted copies changed, unless the pool grew
@@ -1488,8 +1504,12 @@ DevGraph graph_of(const dsm_engine *e, const dsm_engine::GraphCopy &c) {
 // Makes the engine's graph equal to `c` when it lags by exactly the last
 // update (shadow_delta_ok): only that update's touched, dirty and relabelled
 // vertices are copied (launch_sync), plus the signature rows and the apply
-// status; otherwise the whole graph (copy_graph).
-int catch_up(dsm_engine *e, const dsm_engine::GraphCopy &c) {
+// status; otherwise the whole graph (copy_graph). With `deferred`, the dirty
+// vertices' label-sorted copies are left to the update that follows (*deferred
+// is set when there are some to copy; see SortedSync).
+int catch_up(dsm_engine *e, const dsm_engine::GraphCopy &c, SortedSync *deferred = nullptr,
+             bool *has_deferred = nullptr) {
+  if (has_deferred) *has_deferred = false;
   if (!kDeltaSync || !e->shadow_delta_ok || e->pool_cap != c.pool_cap || e->n >= (1u << 31))
     return copy_graph(e, c);
   const uint32_t nt = (uint32_t)e->last_touched.size(), nr = (uint32_t)e->last_relabel.size();
@@ -1504,8 +1524,14 @@ int catch_up(dsm_engine *e, const dsm_engine::GraphCopy &c) {
   // long lists queued: at most one per touched and one per dirty vertex
   HIPCHK(e->sync_q.ensure(2 * (size_t)e->n));
   HIPCHK(e->sync_ctl.ensure(2));
-  HIPCHK(launch_sync(graph_of(e, c), e->dev_graph(), e->d_last_touched.p, nt, e->dirty_list.p, &c.actl.p->ndirty,
-                     e->d_last_relabel.p, nr, e->sync_q.p, (uint32_t)e->sync_q.n, e->sync_ctl.p, e->stream));
+  const bool defer = deferred && has_deferred;
+  HIPCHK(launch_sync(graph_of(e, c), e->dev_graph(), e->d_last_touched.p, nt, defer ? nullptr : e->dirty_list.p,
+                     &c.actl.p->ndirty, e->d_last_relabel.p, nr, e->sync_q.p, (uint32_t)e->sync_q.n, e->sync_ctl.p,
+                     e->stream));
+  if (defer) {
+    *deferred = SortedSync{graph_of(e, c), &c.actl.p->ndirty};
+    *has_deferred = true;
+  }
   HIPCHK(copy_buf(e->bm, c.bm, (size_t)e->nsig_rows * e->words, e->stream));
   HIPCHK(copy_buf(e->actl, c.actl, 1, e->stream));
   return DSM_OK;
@@ -1519,8 +1545,10 @@ int overlap_apply(dsm_engine *e, const HostDelta &d, bool stats, bool keep = tru
   auto run = [&]() -> int {
     HIPCHK(hipStreamWaitEvent(e->stream, e->ev_pre, 0));
     if (stats) HIPCHK(hipEventRecord(e->ev[0], e->stream));
-    if (int r = catch_up(e, e->shadow)) return r;
-    if (int r = apply_delta_and_refresh(e, d)) return r;
+    SortedSync ss;
+    bool deferred = false;
+    if (int r = catch_up(e, e->shadow, &ss, &deferred)) return r;
+    if (int r = apply_delta_and_refresh(e, d, deferred ? &ss : nullptr)) return r;
     if (stats) HIPCHK(hipEventRecord(e->ev[2], e->stream));
     HIPCHK(hipMemcpyAsync(e->h_actl.p, e->actl.p, sizeof(ApplyCtl), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));

@@ -329,11 +329,16 @@ hipError_t launch_apply_commit_failed(const DevGraph &g, const BatchDev &b, cons
 hipError_t launch_dirty_compact(const DirtyArgs &d, uint32_t n, hipStream_t s);
 // Copies what the last update changed from graph copy x to the stale copy y:
 // the touched vertices' slots and lists, the dirty vertices' label-sorted
-// copies (ndirty on the device), the relabelled vertices' labels. q (qcap
-// entries) and qctl[2] are scratch for the long lists.
+// copies (ndirty on the device; none when dirty is null), the relabelled
+// vertices' labels. q (qcap entries) and qctl[2] are scratch for the long
+// lists.
 hipError_t launch_sync(const DevGraph &x, const DevGraph &y, const uint32_t *touched, uint32_t ntouched,
                        const uint32_t *dirty, const uint32_t *ndirty, const uint32_t *relabeled, uint32_t nrelabeled,
                        uint32_t *q, uint32_t qcap, uint32_t *qctl, hipStream_t s);
+// The label-sorted copies and bucket offsets of the listed (dirty) vertices
+// from x to y, except those whose bit is set in skip.
+hipError_t launch_sync_sorted(const DevGraph &x, const DevGraph &y, const uint32_t *dirty, const uint32_t *ndirty,
+                              const uint32_t *skip, uint32_t *q, uint32_t qcap, uint32_t *qctl, hipStream_t s);
 hipError_t launch_relabel(const DevGraph &g, const BatchDev &b, const FilterTables &ft,
                           const DirtyArgs &d, hipStream_t s);
 hipError_t launch_filter_full(const DevGraph &g, const FilterTables &ft, hipStream_t s);

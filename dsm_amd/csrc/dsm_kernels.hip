@@ -1990,10 +1990,11 @@ constexpr uint32_t kSyncWave = 4096;
 
 // mode 0: touched vertices (start, deg, cap and the pool slot); mode 1: dirty
 // vertices (bucket offsets and the label-sorted copy of lists longer than
-// kLabelSorted); mode 2: relabelled vertices (label).
+// kLabelSorted), except those whose bit is set in `skip`; mode 2: relabelled
+// vertices (label).
 __global__ __launch_bounds__(256) void k_sync_small(DevGraph x, DevGraph y, const uint32_t *list, const uint32_t *count,
-                                                    uint32_t n_host, uint32_t mode, uint32_t *q, uint32_t qcap,
-                                                    uint32_t *qctl) {
+                                                    uint32_t n_host, uint32_t mode, const uint32_t *skip, uint32_t *q,
+                                                    uint32_t qcap, uint32_t *qctl) {
   const uint32_t lane = lane_id();
   const uint32_t total = count ? (*count < x.n ? *count : x.n) : n_host;
   for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < total; t += (gridDim.x * blockDim.x) >> 6) {
@@ -2002,6 +2003,7 @@ __global__ __launch_bounds__(256) void k_sync_small(DevGraph x, DevGraph y, cons
       if (lane == 0) y.lab[v] = x.lab[v];
       continue;
     }
+    if (skip && ((skip[v >> 5] >> (v & 31)) & 1u)) continue;
     const uint32_t st = x.start[v], d = x.deg[v];
     if (mode == 0) {
       if (lane == 0) {
@@ -2092,18 +2094,32 @@ hipError_t launch_sync(const DevGraph &x, const DevGraph &y, const uint32_t *tou
   hipLaunchKernelGGL(k_sync_reset, dim3(1), dim3(64), 0, s, qctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  const uint32_t *no_skip = nullptr;
   if (ntouched) {
     hipLaunchKernelGGL(k_sync_small, dim3((ntouched + 3) / 4 < 4096 ? (ntouched + 3) / 4 : 4096), dim3(256), 0, s, x, y,
-                       touched, (const uint32_t *)nullptr, ntouched, 0u, q, qcap, qctl);
+                       touched, (const uint32_t *)nullptr, ntouched, 0u, no_skip, q, qcap, qctl);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_sync_small, dim3(4096), dim3(256), 0, s, x, y, dirty, ndirty, 0u, 1u, q, qcap, qctl);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (dirty) {
+    hipLaunchKernelGGL(k_sync_small, dim3(4096), dim3(256), 0, s, x, y, dirty, ndirty, 0u, 1u, no_skip, q, qcap, qctl);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   if (nrelabeled) {
     hipLaunchKernelGGL(k_sync_small, dim3((nrelabeled + 3) / 4 < 4096 ? (nrelabeled + 3) / 4 : 4096), dim3(256), 0, s, x, y,
-                       relabeled, (const uint32_t *)nullptr, nrelabeled, 2u, q, qcap, qctl);
+                       relabeled, (const uint32_t *)nullptr, nrelabeled, 2u, no_skip, q, qcap, qctl);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  hipLaunchKernelGGL(k_sync_big, dim3(1024), dim3(256), 0, s, x, y, q, qcap, qctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_sync_sorted(const DevGraph &x, const DevGraph &y, const uint32_t *dirty, const uint32_t *ndirty,
+                              const uint32_t *skip, uint32_t *q, uint32_t qcap, uint32_t *qctl, hipStream_t s) {
+  hipLaunchKernelGGL(k_sync_reset, dim3(1), dim3(64), 0, s, qctl);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sync_small, dim3(4096), dim3(256), 0, s, x, y, dirty, ndirty, 0u, 1u, skip, q, qcap, qctl);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_sync_big, dim3(1024), dim3(256), 0, s, x, y, q, qcap, qctl);
   return hipGetLastError();
 }
