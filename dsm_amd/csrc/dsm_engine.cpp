@@ -1283,12 +1283,6 @@ void take_refine_time(dsm_engine *e) {
   if (hipEventElapsedTime(&ms, e->ev_ref[0], e->ev_ref[1]) == hipSuccess) e->stats.ms_refine += ms;
 }
 
-// One batch's update and candidate refresh from its host delta, enqueued on
-// the engine's stream: reset the per-batch apply counters, apply (device
-// merge/commit, which waits for the commit's status), relabel, rebuild the
-// dirty vertices' label-sorted copies, then refilter the dirty vertices (long
-// lists take their neighbour-label counts from those copies). No wait at the
-// end.
 // The label-sorted copies that a catch-up leaves to the update that follows
 // it: those of the last update's dirty vertices (the shared dirty list still
 // holds them until the next compaction), copied from `from` once the new
@@ -1299,6 +1293,12 @@ struct SortedSync {
   const uint32_t *ndirty;
 };
 
+// One batch's update and candidate refresh from its host delta, enqueued on
+// the engine's stream: reset the per-batch apply counters, apply (device
+// merge/commit, which waits for the commit's status), relabel, make a
+// catch-up's deferred label-sorted copies (sorted_sync), rebuild the dirty
+// vertices' label-sorted copies, then refilter the dirty vertices (long lists
+// take their neighbour-label counts from those copies). No wait at the end.
 int apply_delta_and_refresh(dsm_engine *e, const HostDelta &d, const SortedSync *sorted_sync = nullptr) {
   int rc;
   e->shadow_delta_ok = false;
@@ -1505,8 +1505,9 @@ DevGraph graph_of(const dsm_engine *e, const dsm_engine::GraphCopy &c) {
 // update (shadow_delta_ok): only that update's touched, dirty and relabelled
 // vertices are copied (launch_sync), plus the signature rows and the apply
 // status; otherwise the whole graph (copy_graph). With `deferred`, the dirty
-// vertices' label-sorted copies are left to the update that follows (*deferred
-// is set when there are some to copy; see SortedSync).
+// vertices' label-sorted copies are left to the update that follows: when the
+// copy is by the last update only, *deferred describes them and *has_deferred
+// is set (see SortedSync).
 int catch_up(dsm_engine *e, const dsm_engine::GraphCopy &c, SortedSync *deferred = nullptr,
              bool *has_deferred = nullptr) {
   if (has_deferred) *has_deferred = false;
