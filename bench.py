@@ -450,11 +450,14 @@ def main():
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-query plain leg (rmat1m: one 6-vertex query, one batch per call)")
     ap.add_argument("--tune-sample", default="edges", choices=["edges", "batches"],
-                    help="the autotuner's sample: 16,384 random edges of the initial graph, or the records of "
-                         "the first 4 batches")
+                    help="the autotuner's sample: random edges of the initial graph (--tune-records of them), or "
+                         "the records of the first 4 batches")
+    ap.add_argument("--tune-records", type=int, default=TUNE_SAMPLE["records"],
+                    help="random edges in the autotuner's sample")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="only run one D2D copy of this many bytes")
     args = ap.parse_args()
     TUNE_SAMPLE["kind"] = args.tune_sample
+    TUNE_SAMPLE["records"] = args.tune_records
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
