@@ -808,3 +808,59 @@ def test_second_copy_catch_up_paths():
                 eng.autotune(s_add, s_src, s_dst, s_lab)
     finally:
         eng.close()
+
+
+def test_label_sorted_copies_dirty_in_consecutive_overlapped_updates():
+    """The second copy's label-sorted lists when hubs are relabelled batch
+    after batch: an overlapped update copies the previous update's dirty
+    copies into its graph only after marking its own dirty vertices, and skips
+    those, whose copies it rebuilds. Hub 0 is relabelled in every batch (its
+    long-list neighbours are dirty in consecutive updates); hubs 1-2 and 3-4
+    in alternate batches (their neighbours are dirty in one update and clean
+    in the next, so their copies are copied). Dense graph, so most lists have
+    label-sorted copies; pipelined batches against the oracle per batch."""
+    say("case: label-sorted copies dirty in consecutive overlapped updates")
+    rng = np.random.default_rng(7373)
+    n, hubs = 1500, 5
+    labels, eu, ev = random_graph(n, 45000, 3, rng)
+    edges = set(zip(eu.tolist(), ev.tolist()))
+    for h in range(hubs):
+        for v in rng.choice(np.arange(hubs, n), 500, replace=False):
+            edges.add((h, int(v)))
+    e = np.array(sorted(edges), np.uint32)
+    eu, ev = e[:, 0].copy(), e[:, 1].copy()
+    present = set(edges)
+    queries = [qtuple("triangle", [0, 1, 2]), qtuple("tri_pendant", [1, 0, 2, 0])]
+    batches = []
+    for bi in range(8):
+        add, src, dst, lab = conflict_free_batch(n, 150, present, rng)
+        pairs = set(zip(np.minimum(src, dst).tolist(), np.maximum(src, dst).tolist()))
+        relabelled = [0] + ([1, 2] if bi % 2 == 0 else [3, 4])
+        extra = []
+        for h in relabelled:
+            while True:
+                v = int(rng.integers(hubs, n))
+                if (h, v) not in present and (h, v) not in pairs:
+                    break
+            present.add((h, v))
+            pairs.add((h, v))
+            extra.append((h, v, (bi + h) % 3))
+        add = np.concatenate([add, np.ones(len(extra), np.uint8)])
+        src = np.concatenate([src, np.array([x[0] for x in extra], np.uint32)])
+        dst = np.concatenate([dst, np.array([x[1] for x in extra], np.uint32)])
+        lab = np.concatenate([lab, np.array([x[2] for x in extra], np.uint32)])
+        batches.append((add, src, dst, lab))
+    g = O.Graph(n, labels, eu, ev)
+    want = O.run_stream(g, oracle_queries(queries), batches, relabel=True)
+    say("  oracle done")
+    eng = E.Engine((n, labels, eu, ev), queries, relabel=True, overlap_apply=True)
+    try:
+        pos, neg, _ = eng.process_batches(batches)
+        for b in range(len(batches)):
+            assert pos[b].tolist() == want[b]["pos"], "batch %d positives" % b
+            assert neg[b].tolist() == want[b]["neg"], "batch %d negatives" % b
+        off, nb, _ = eng.graph()
+        ooff, onb, _ = g.export()
+        assert off.tolist() == ooff.tolist() and np.array_equal(nb, onb)
+    finally:
+        eng.close()
